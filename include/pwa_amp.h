@@ -1,0 +1,265 @@
+/*
+ * pwa_amp.h -- C ABI of the MI355X Lambda_b -> J/psi p K- amplitude likelihood.
+ *
+ * This is the drop-in seam between the fit-side host code (a RooDalitzAmplitude-
+ * style PDF and a Minuit-style FCN) and the HIP kernels that evaluate the
+ * unbinned negative log-likelihood.  It replaces the C++ API of the reference,
+ * RooTDalitz/kernel_calEva.h:19-26 (cu_malloc_h_pp, host_store_fx,
+ * cuda_configure_*, host_store_amp2s, cu_Free_pp), whose state lived in
+ * process-global statics and __constant__ symbols (src/kernel_calEva.cu:41-80).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: PWA_OK (0) or a negative PWA_E_*
+ *    code; pwa_last_error() returns a text for the last failure on a context.
+ *  - A context owns all device memory and is bound to one HIP device and one
+ *    stream; the caller owns every host array it passes.  Calls on one context
+ *    are not thread-safe; distinct contexts are independent.
+ *  - All arithmetic is IEEE FP64.
+ *  - No torch or HIP types appear in any signature.
+ */
+#ifndef PWA_AMP_H
+#define PWA_AMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PWA_OK 0
+#define PWA_E_ARG (-1)      /* invalid argument (null pointer, bad size, bad id)        */
+#define PWA_E_MODEL (-2)    /* inconsistent model description                             */
+#define PWA_E_HIP (-3)      /* HIP runtime failure (message in pwa_last_error)            */
+#define PWA_E_STATE (-4)    /* call out of order (e.g. eval before events are uploaded)   */
+#define PWA_E_NOMEM (-5)    /* device allocation failed                                   */
+
+/* --------------------------------------------------------- model encoding */
+/* Resonance chains, in the order the parameter vector lists them
+   (src/RooDalitzAmplitude.cxx:3068-3139: Lambda*, then Pc, then Zcs).   */
+#define PWA_CHAIN_LAMBDA 0 /* K- p        resonances, lineshape in m(Kp)      */
+#define PWA_CHAIN_PC 1     /* J/psi p     resonances, lineshape in m(J/psi p) */
+#define PWA_CHAIN_ZCS 2    /* J/psi K-    resonances, lineshape in m(J/psi K) */
+
+/* Lineshape type codes: the reference's TYPE enum (RooTDalitz/Resonances.h:108-120). */
+#define PWA_BREITWIGNER 1
+#define PWA_KMATRIX 240
+#define PWA_NONRESONANT 361
+
+#define PWA_MAX_RES 64        /* resonances per model                              */
+#define PWA_MAX_SCHEMES 8     /* smearing schemes (scheme id 1..n_schemes)         */
+#define PWA_MAX_SUBBINS 160   /* sub-bins per smearing scheme                      */
+#define PWA_KM_NCH 4          /* K-matrix channels (RooTDalitz/Resonances.h:6)      */
+#define PWA_KM_NPOLE 4        /* K-matrix poles per J^P class (Resonances.h:106)    */
+#define PWA_KM_NCLASS 4       /* J^P classes: 0 = 1/2-, 1 = 3/2-, 2 = 1/2+, 3 = 3/2+ */
+
+/* Behaviour switches (pwa_model_desc.flags).  Each one reproduces a reference
+   choice that changes results; all are on in the reference preset.          */
+#define PWA_F_NEG_FX_PENALTY 0x1u  /* data event with |A|^2 <= 0 adds 1e12 (src/kernel_calEva.cu:964-967) */
+#define PWA_F_NOINT_PENALTY 0x2u   /* +1000 if |I_noint/I - 1| > 0.3 (src/kernel_calEva.cu:1360-1361)      */
+#define PWA_F_KM_CLAMP 0x4u        /* |m0^2 - s| < 1e-3 -> 1e-3 in the K-matrix (src/cu_DPFPropogator.cu:735-736) */
+/* Generalisation (off in the reference preset): the i-th non-resonant K-matrix
+   member of a class (i >= 1) also carries f[i][i..3] in its slots 9.., so the
+   whole symmetric background matrix floats.  The reference fills only row and
+   column 0 (src/RooDalitzAmplitude.cxx:2902-2916).                           */
+#define PWA_F_KM_FULL_F 0x8u
+
+/* Number of per-event input columns (see PWA_COL_*). */
+#define PWA_NCOL 24
+
+/* Per-event input columns, in the reference ntuple's branch meaning
+   (src/execute.cpp:33-57, src/RooDalitzAmplitude.cxx:327-354).  Masses in GeV,
+   angles in radians, cosines in [-1, 1].                                      */
+enum {
+    PWA_COL_MKP = 0,       /* mLk            m(K- p)                          */
+    PWA_COL_MJPSIP,        /* mjpsiL         m(J/psi p)                       */
+    PWA_COL_MJPSIK,        /* mjpsik         m(J/psi K-)                      */
+    PWA_COL_COS_LB,        /* cosTheta_Lb    Lambda* chain                    */
+    PWA_COL_COS_L,         /* cosTheta_L                                      */
+    PWA_COL_COS_JPSI,      /* cosTheta_Jpsi                                   */
+    PWA_COL_PHI_K,         /* phiK                                            */
+    PWA_COL_PHI_MU,        /* phiMu                                           */
+    PWA_COL_Z_COS_LB,      /* Z_cosTheta_Lb  Pc chain                         */
+    PWA_COL_Z_COS_Z,       /* Z_cosTheta_Z                                    */
+    PWA_COL_Z_COS_JPSI,    /* Z_cosTheta_Jpsi                                 */
+    PWA_COL_Z_PHI_Z,       /* Z_phiZ                                          */
+    PWA_COL_Z_PHI_JPSI,    /* Z_phiJpsi                                       */
+    PWA_COL_Z_PHI_MU,      /* Z_phiMu                                         */
+    PWA_COL_COS_P,         /* cosTheta_Lambda (proton alignment angle)        */
+    PWA_COL_ALPHA_MU,      /* alpha_Mu       (muon alignment angle)           */
+    PWA_COL_X_COS_LB,      /* X_cosTheta_Lb  Zcs chain                        */
+    PWA_COL_X_COS_X,       /* X_cosTheta_X                                    */
+    PWA_COL_X_COS_JPSI,    /* X_cosTheta_Jpsi                                 */
+    PWA_COL_X_PHI_X,       /* X_phiX                                          */
+    PWA_COL_X_PHI_JPSI,    /* X_phiJpsi                                       */
+    PWA_COL_X_PHI_MU,      /* X_phiMu                                         */
+    PWA_COL_X_COS_P,       /* X_cosTheta_p                                    */
+    PWA_COL_X_ALPHA_MU     /* X_alpha_Mu                                      */
+};
+
+/* Event kinds for pwa_upload_events. */
+#define PWA_DATA 0
+#define PWA_MC 1
+
+/*
+ * Model description: the runtime replacement of the reference's compile-time
+ * resonance counts (RooTDalitz/NDefine.h:5-14), its per-resonance setup
+ * (setup_paraList, src/RooDalitzAmplitude.cxx:3068-3139) and its constant
+ * tables (RooTDalitz/Resonances.h:5-120, src/cu_DPFPropogator.cu:11-15).
+ * Resonance r is Lambda* for r < n_lambda, Pc for r < n_lambda+n_pc, else Zcs.
+ */
+typedef struct pwa_model_desc {
+    int32_t n_lambda, n_pc, n_zcs;
+    int32_t type[PWA_MAX_RES];   /* lineshape type code (PWA_BREITWIGNER, ...) */
+    int32_t spin2[PWA_MAX_RES];  /* 2J                                        */
+    int32_t parity[PWA_MAX_RES]; /* +1 or -1                                  */
+    double m_lb, m_p, m_k, m_jpsi; /* masses used by the amplitude (GeV)      */
+    double ffr, ffb;             /* Blatt-Weisskopf radii: resonance, Lambda_b vertex (GeV^-1) */
+    /* K-matrix phase-space channels per J^P class: channel c couples masses
+       (km_ma, km_mb - i*km_hw); km_cplx selects the complex-mass phase space.
+       Channel 0 is J/psi p.  (src/cu_DPFPropogator.cu:588-680)             */
+    double km_ma[PWA_KM_NCLASS][PWA_KM_NCH];
+    double km_mb[PWA_KM_NCLASS][PWA_KM_NCH];
+    double km_hw[PWA_KM_NCLASS][PWA_KM_NCH];
+    int32_t km_cplx[PWA_KM_NCLASS][PWA_KM_NCH];
+    /* Channels used for the near-threshold pole-mass shift, by the channel
+       index a K-matrix pole carries in its width slot (1..3; 0 = no shift).
+       (src/cu_DPFPropogator.cu:695-728)                                     */
+    double km_shift_ma[PWA_KM_NCH], km_shift_mb[PWA_KM_NCH], km_shift_hw[PWA_KM_NCH];
+    /* Resolution smearing of the Pc lineshapes in m(J/psi p): scheme s (1-based
+       id) evaluates the lineshape at m + z_b * sigma(m) for its sub-bins b with
+       weight 1/nbins, sigma(m) = mres[0] + m*(mres[1] + m*mres[2]).
+       (src/kernel_calEva.cu:664-689, RooTDalitz/Resonances.h:31-58)          */
+    int32_t n_schemes;
+    int32_t scheme_nbins[PWA_MAX_SCHEMES];
+    double scheme_z[PWA_MAX_SCHEMES][PWA_MAX_SUBBINS];
+    double mres[3];
+    uint32_t flags;     /* PWA_F_* */
+    double nll_shift;   /* NLL_SHIFT added to the NLL (Makefile:13)           */
+} pwa_model_desc;
+
+/*
+ * Fit-parameter vector: the concatenation of every resonance's parameter list
+ * in the reference's RooArgList layout (Lambda*, then Pc, then Zcs):
+ *   Lambda*: [ 6 x (Re,Im) LS couplings, scRe, scIm, m0, width, type, 2J, P ]   19
+ *            (RooTDalitz/ResL_fullPHSP_finnerBinning.h:22-37)
+ *   Pc     : [ a_Bp, p_Bp, 3 x (Re,Im) LS, g1..g4 (or fsc in slot 8), m0,
+ *              width-or-channel, type, 2J, P ]                                17
+ *            (RooTDalitz/ResZKM_alternateCoupling.h:18-43)
+ *   Zcs    : [ 4 x (Re,Im) LS, a_Bp, p_Bp, m0, width, type, 2J, P ]           15
+ *            (RooTDalitz/ResX.h:1-28)
+ * The type/2J/P entries must agree with the model description.
+ */
+#define PWA_NPAR_LAMBDA 19
+#define PWA_NPAR_PC 17
+#define PWA_NPAR_ZCS 15
+
+typedef struct pwa_ctx pwa_ctx;
+
+/* Size of the fit-parameter vector for a model. */
+int pwa_fit_param_count(const pwa_model_desc* model, int32_t* n_params);
+
+/* Create a context on HIP device `device`.  Replaces the constructor-time
+   cuda_configure_Pc_KMatrix / cuda_configure_ampcount / cuda_configure_smear
+   (RooTDalitz/kernel_calEva.h:21-23, called at src/RooDalitzAmplitude.cxx:519-521). */
+int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out);
+
+/* Destroy a context and free its device memory (replaces cu_Free_pp,
+   RooTDalitz/kernel_calEva.h:19 and the destructor at RooDalitzAmplitude.h:117). */
+int pwa_destroy(pwa_ctx* ctx);
+
+/* Text of the last error on this context (never NULL). */
+const char* pwa_last_error(const pwa_ctx* ctx);
+
+/*
+ * Upload one event sample (replaces the six cu_malloc_h_pp uploads of the
+ * per-event SoA planes, src/RooDalitzAmplitude.cxx:557-770 and
+ * src/kernel_calEva.cu:1447-1457).
+ *   kind    : PWA_DATA or PWA_MC (each kind is uploaded once; a new upload replaces it)
+ *   n       : number of events
+ *   cols    : column-major block, column c (PWA_COL_*) at cols[c*ld + i]
+ *   ld      : leading dimension (>= n)
+ *   weight  : data: sWeight per event; MC: per-event weight (<= -1000 means 1)
+ *   scheme  : per-event smearing scheme id (0 = none, 1..n_schemes), or NULL for none
+ * The events are reordered on the device by scheme; per-event outputs are
+ * returned in the caller's order.
+ */
+int pwa_upload_events(pwa_ctx* ctx, int kind, int64_t n, const double* cols, int64_t ld,
+                      const double* weight, const int32_t* scheme);
+
+/*
+ * One likelihood evaluation at fit-parameter vector `params` (the
+ * host_store_fx equivalent, RooTDalitz/kernel_calEva.h:20,
+ * src/kernel_calEva.cu:1186-1363, with the coupling map of
+ * src/RooDalitzAmplitude.cxx:2669-2986 done on the host).
+ *   nll      : NLL_SHIFT + sumW_data*ln(I/sumW_mc) - sum_data sw*ln|A|^2 (+ penalties)
+ *   integral : I = sum_mc w*|A|^2          (may be NULL)
+ *   fx_data  : per-data-event |A|^2        (may be NULL, caller order)
+ *   fx_mc    : per-MC-event w*|A|^2        (may be NULL, caller order)
+ */
+int pwa_eval(pwa_ctx* ctx, const double* params, double* nll, double* integral,
+             double* fx_data, double* fx_mc);
+
+/*
+ * Likelihood and its analytic gradient with respect to every entry of the
+ * fit-parameter vector (entries that are not continuous parameters -- type,
+ * 2J, P, K-matrix channel indices -- get 0).  No reference counterpart: the
+ * reference gives Minuit only values (src/Minuit3Minimizer.cxx:398-416 is the
+ * gradient interface it leaves unused).
+ */
+int pwa_eval_grad(pwa_ctx* ctx, const double* params, double* nll, double* grad);
+
+/*
+ * Fit-level objective, as Minuit sees it (src/fit.cpp:312-317):
+ *   fcn = (nll - NLL_SHIFT) * alpha + sum_c penalty_c(x_{idx_c})
+ * with the one-sided Gaussian penalty of src/RooFGaussian.cxx:60-75,
+ *   0.5*((x-mu1)/s1)^2 for x < mu1, 0.5*((x-mu2)/s2)^2 for x > mu2, else 0.
+ * alpha = sum sw / sum sw^2 over data (src/fit.cpp:201-220) is computed from
+ * the uploaded data weights.  grad may be NULL (value only).
+ */
+int pwa_set_constraints(pwa_ctx* ctx, int32_t n, const int32_t* idx, const double* mu1,
+                        const double* mu2, const double* s1, const double* s2);
+int pwa_fcn(pwa_ctx* ctx, const double* params, double* fcn, double* grad);
+
+/* alpha = sum sw / sum sw^2 of the uploaded data (src/fit.cpp:201-220). */
+int pwa_alpha(pwa_ctx* ctx, double* alpha);
+
+/* Per-event |A|^2 without smearing (replaces host_store_amp2s,
+   RooTDalitz/kernel_calEva.h:24, src/kernel_calEva.cu:1365-1419). */
+int pwa_amp2(pwa_ctx* ctx, int kind, const double* params, double* amp2);
+
+/* Device time of the last pwa_eval / pwa_eval_grad kernels, in ms (HIP events
+   on the context's stream), and the name of the dominant kernel.          */
+int pwa_last_timing(pwa_ctx* ctx, double* ms_total, double* ms_main);
+
+/*
+ * Synthetic events: uniform Lambda_b -> J/psi p K- phase space with
+ * J/psi -> mu+ mu-, boosted to a lab frame, converted to the PWA_COL_* helicity
+ * variables with the conventions of gen_toys/phsp_toys/tree.C:127-408
+ * (generator: gen_toys/phsp_toys/gentoy_formhj.C:10-115).  Deterministic in
+ * `seed`; cols is column-major with leading dimension n.
+ */
+int pwa_generate_phsp(uint64_t seed, int64_t n, double m_lb, double m_jpsi, double m_p,
+                      double m_k, double m_mu, double* cols);
+
+/* The same generator, returning lab four-vectors: 16 doubles per event,
+   (px, py, pz, E) of mu+, mu-, p, K- in that order. */
+int pwa_generate_phsp_p4(uint64_t seed, int64_t n, double m_lb, double m_jpsi, double m_p,
+                         double m_k, double m_mu, double* p4);
+
+/* Helicity variables (PWA_COL_* columns, column-major, leading dimension n)
+   of events given as lab four-vectors in the pwa_generate_phsp_p4 layout
+   (gen_toys/phsp_toys/tree.C:127-408). */
+int pwa_phsp_angles(int64_t n, const double* p4, double* cols);
+
+/* The LS -> helicity coupling map on the host (ConfigureCouplings<true>,
+   src/RooDalitzAmplitude.cxx:2669-2986): writes the complex helicity
+   couplings (interleaved re, im) in the reference's _vamp order and their
+   count; `out` may be NULL to query the count. */
+int pwa_helicity_couplings(const pwa_model_desc* model, const double* params, double* out,
+                           int32_t* n_couplings);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PWA_AMP_H */

@@ -1,0 +1,151 @@
+"""ORACLE -- test infrastructure only (the checker, never the product).
+
+ctypes front-end of liboracle.so, the CPU FP64 restatement of the
+reference amplitude likelihood (oracle/pwa_oracle.cpp).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "orc_dfun": (C.c_double, [C.c_double] * 4),
+            "orc_dfunhalf": (C.c_double, [C.c_double, C.c_int, C.c_int]),
+            "orc_dfunone": (C.c_double, [C.c_double, C.c_int, C.c_int]),
+            "orc_cg": (C.c_double, [C.c_int] * 6),
+            "orc_c2": (C.c_double, [C.c_int, C.c_int]),
+            "orc_getp": (C.c_double, [C.c_double] * 3),
+            "orc_fbw": (C.c_double, [C.c_double, C.c_int, C.c_double, C.c_double]),
+            "orc_getfbw2": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]),
+            "orc_csqrt": (None, [C.c_double, C.c_double, _dp]),
+            "orc_breit_wigner": (None, [C.c_double, C.c_double, C.c_double, _dp]),
+            "orc_bw_amp": (None, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+            "orc_bterm": (C.c_double, [vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+            "orc_solve_row0": (None, [_dp, C.c_int, _dp]),
+            "orc_fit_param_count": (C.c_int, [vp]),
+            "orc_build_amp": (C.c_int, [vp, _dp, _dp, _dp, _ip, _dp, C.c_int, _dp, _ip]),
+            "orc_kmatrix": (C.c_int, [vp, _dp, C.c_double, C.c_int, _dp]),
+            "orc_filldlz": (None, [_dp, _dp]),
+            "orc_amp2": (C.c_int, [vp, _dp, C.c_int64, _dp, C.c_int64, _ip, C.c_int, _dp]),
+            "orc_nll": (C.c_int, [vp, _dp, C.c_int64, _dp, C.c_int64, _dp, _ip, C.c_int64, _dp, C.c_int64, _dp,
+                                  _ip, _dp, _dp, _dp, _dp, _dp]),
+            "orc_reference_schemes": (None, [C.c_int64, _dp, C.c_int64, C.c_int64, C.c_int64, _dp, _ip, _ip]),
+            "orc_fcn": (C.c_double, [C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
+            "orc_alpha": (C.c_double, [C.c_int64, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def ip(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def dref(desc):
+    return C.cast(C.pointer(desc), C.c_void_p)
+
+
+def dfun(beta, j, m, n):
+    return lib().orc_dfun(beta, j, m, n)
+
+
+def cg(J, M, j1, j2, m1, m2):
+    return lib().orc_cg(J, M, j1, j2, m1, m2)
+
+
+def build_amp(desc, params, cap=4096):
+    n = desc.n_res
+    m0, g0 = np.zeros(n), np.zeros(n)
+    L = np.zeros(n, dtype=np.int32)
+    ampc = np.zeros(n, dtype=np.int32)
+    v = np.zeros(2 * cap)
+    km = np.zeros(128)
+    p = _d(params)
+    nv = lib().orc_build_amp(dref(desc), dp(p), dp(m0), dp(g0), ip(L), dp(v), cap, dp(km), ip(ampc))
+    if nv < 0:
+        raise RuntimeError("orc_build_amp failed")
+    return dict(m0=m0, g0=g0, L=L, vamp=v[0:2 * nv:2] + 1j * v[1:2 * nv:2], km=km.reshape(4, 32), ampcount=ampc)
+
+
+def amp2(desc, params, cols, scheme=None, interference=True):
+    cols = _d(cols)
+    n = cols.shape[1]
+    out = np.zeros(n)
+    sc = _i(scheme)
+    rc = lib().orc_amp2(dref(desc), dp(_d(params)), n, dp(cols), n, ip(sc), 1 if interference else 0, dp(out))
+    if rc:
+        raise RuntimeError("orc_amp2 failed")
+    return out
+
+
+def nll(desc, params, dcols, sw, dscheme, mcols, wmc, mscheme):
+    dcols, mcols = _d(dcols), _d(mcols)
+    nd, nm = dcols.shape[1], mcols.shape[1]
+    out = np.zeros(3)
+    fxd, fxm = np.zeros(nd), np.zeros(nm)
+    v, I, In = C.c_double(), C.c_double(), C.c_double()
+    rc = lib().orc_nll(dref(desc), dp(_d(params)), nd, dp(dcols), nd, dp(_d(sw)), ip(_i(dscheme)), nm, dp(mcols), nm,
+                       dp(_d(wmc)), ip(_i(mscheme)), C.cast(C.pointer(v), _dp), C.cast(C.pointer(I), _dp),
+                       C.cast(C.pointer(In), _dp), dp(fxd), dp(fxm))
+    if rc:
+        raise RuntimeError("orc_nll failed")
+    return dict(nll=v.value, integral=I.value, integral_noint=In.value, fx_data=fxd, fx_mc=fxm)
+
+
+def kmatrix(desc, params, mpp, ik):
+    out = np.zeros(16)
+    lib().orc_kmatrix(dref(desc), dp(_d(params)), mpp, ik, dp(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def filldlz(ev):
+    out = np.zeros(3 + 48 + 60 + 96 + 96 + 96 + 96)
+    lib().orc_filldlz(dp(_d(ev)), dp(out))
+    return out
+
+
+def alpha(w):
+    w = _d(w)
+    return lib().orc_alpha(len(w), dp(w))
+
+
+def fcn(nll_value, nll_shift, alpha_value, params, idx, mu1, mu2, s1, s2):
+    idx = _i(idx)
+    return lib().orc_fcn(nll_value, nll_shift, alpha_value, dp(_d(params)), len(idx), ip(idx), dp(_d(mu1)),
+                         dp(_d(mu2)), dp(_d(s1)), dp(_d(s2)))

@@ -1,0 +1,1301 @@
+// ORACLE -- test infrastructure only.
+//
+// A CPU FP64 restatement of the reference's amplitude likelihood
+// (userzxleo-cmyk/PWA-Xib2JpsiLambdaK) used as the checker for the HIP path.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// it; the product library never links or calls it.
+//
+// Every function names the reference file:line it restates.  The reference
+// computes in FP32 (float2 tables and couplings); this restatement keeps the
+// reference's algorithm and branch structure but evaluates everything in
+// IEEE FP64.  Where the reference's arithmetic lives in a third-party library
+// (ROOT TMath/TComplex, CUDA math) the mathematical function is called
+// (std::cos, std::sqrt, std::lgamma ...), never re-derived.
+//
+// Pinning (tests/test_oracle_*.py): Wigner-d and Clebsch-Gordan against sympy
+// exact values; the reference-held C2 table; Blatt-Weisskopf F(p0)=1 and the
+// von Hippel-Quigg polynomials; Breit-Wigner value at s=m^2; complex sqrt vs
+// the principal branch; K-matrix row-0 against numpy dense solves and the
+// single-channel single-pole Breit-Wigner limit; smearing with sigma->0 equal
+// to the unsmeared amplitude; NLL invariance under a global complex rescaling
+// of all couplings; phase-space angles against an independent numpy
+// restatement of tree.C.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/pwa_amp.h"
+
+namespace {
+
+using cd = std::complex<double>;
+const double kPi = 3.14159265358979323846;
+
+// ------------------------------------------------------------------ Wigner d
+// src/RooDalitzAmplitude.cxx:2443-2455 (dfunhalf)
+double dfunhalf(double beta, int am, int an) {
+    if ((am == 1 && an == 1) || (am == -1 && an == -1)) return std::cos(0.5 * beta);
+    if (am == 1 && an == -1) return -std::sin(0.5 * beta);
+    if (am == -1 && an == 1) return std::sin(0.5 * beta);
+    return 0.0;
+}
+
+// src/RooDalitzAmplitude.cxx:2457-2470 (dfunone)
+double dfunone(double beta, int am, int an) {
+    if ((am == 1 && an == 1) || (am == -1 && an == -1)) return (std::cos(beta) + 1.0) / 2.;
+    if ((am == 1 && an == -1) || (am == -1 && an == 1)) return (-std::cos(beta) + 1.0) / 2.;
+    if (am == 0 && an == 1) return std::sin(beta) * 7.07106781186547461715e-01;
+    if (am == 0 && an == -1) return -std::sin(beta) * 7.07106781186547461715e-01;
+    if (am == 0 && an == 0) return std::cos(beta);
+    return 0.0;
+}
+
+double lfact(int n) { return std::lgamma(n + 1.0); }
+
+// src/RooDalitzAmplitude.cxx:2472-2590 (dfun, CERNLIB DDJMNB, Varshalovich
+// 4.3.1(3)); the reference's log-factorial table fcl[n] is ln(n!) = lgamma(n+1).
+double dfun(double beta, double aj, double am, double an) {
+    const double pi = 3.141592653589793116;
+    const double pi2 = 6.283185307179586232;
+    int jpm = (int)std::lround(aj + am);
+    int jpn = (int)std::lround(aj + an);
+    int jmm = (int)std::lround(aj - am);
+    int jmn = (int)std::lround(aj - an);
+    int mpn = (int)std::lround(am + an);
+    double r = 0;
+    if (jpm < 0 || jpn < 0 || jmm < 0 || jmn < 0 || aj < 0 || aj > 25 || beta < 0 || beta > pi2) {
+        r = 0;
+    } else if (std::fabs(beta) < 1e-10) {
+        if (jpm == jpn) r = 1;
+    } else if (std::fabs(beta - pi) < 1e-10) {
+        if (jpm == jmn) {
+            r = 1;
+            if (std::abs(jpm) % 2 == 1) r = -1;
+        }
+    } else if (std::fabs(beta - pi2) < 1e-10) {
+        if (jpm == jpn) {
+            r = 1;
+            if (std::abs(mpn) % 2 == 1) r = -1;
+        }
+    } else {
+        double b = 0.5 * beta;
+        double s = std::log(std::sin(b));
+        double c = std::log(std::fabs(std::cos(b)));
+        double rt = 0.5 * (lfact(jpm) + lfact(jmm) + lfact(jpn) + lfact(jmn));
+        int k0 = std::max(0, mpn);
+        int kq = k0 + jpm;
+        if (beta > pi) kq += mpn;
+        double q = 1;
+        if (((kq % 2) + 2) % 2 == 1) q = -1;
+        kq = k0 + k0;
+        double cx = kq - mpn;
+        double sx = jpm + jpn - kq;
+        for (int k = k0; k <= std::min(jpm, jpn); k++) {
+            r += q * std::exp(rt - lfact(k) - lfact(jpm - k) - lfact(jpn - k) - lfact(k - mpn) + cx * c + sx * s);
+            cx += 2;
+            sx -= 2;
+            q = -q;
+        }
+    }
+    return r;
+}
+
+// -------------------------------------------------------- Clebsch-Gordan
+// <j1 m1; j2 m2 | J M>, all arguments twice the quantum number: the value of
+// EvtCGCoefSingle(j1,j2).coef(J,M,j1,j2,m1,m2) (src/EvtCGCoefSingle.cpp),
+// here from the Racah formula.
+double cg(int J, int M, int j1, int j2, int m1, int m2) {
+    if (m1 + m2 != M) return 0.0;
+    if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(M) > J) return 0.0;
+    if (J < std::abs(j1 - j2) || J > j1 + j2) return 0.0;
+    if (((j1 + m1) & 1) || ((j2 + m2) & 1) || ((J + M) & 1) || ((j1 + j2 + J) & 1)) return 0.0;
+    const int a = (j1 + j2 - J) / 2, b = (j1 - j2 + J) / 2, c = (-j1 + j2 + J) / 2;
+    const int d = (j1 + j2 + J) / 2 + 1;
+    double lpref = 0.5 * (std::log(J + 1.0) + lfact(a) + lfact(b) + lfact(c) - lfact(d) +
+                          lfact((J + M) / 2) + lfact((J - M) / 2) + lfact((j1 - m1) / 2) +
+                          lfact((j1 + m1) / 2) + lfact((j2 - m2) / 2) + lfact((j2 + m2) / 2));
+    double sum = 0;
+    for (int k = 0; k <= 64; ++k) {
+        const int t1 = a - k, t2 = (j1 - m1) / 2 - k, t3 = (j2 + m2) / 2 - k;
+        const int t4 = (J - j2 + m1) / 2 + k, t5 = (J - j1 - m2) / 2 + k;
+        if (t1 < 0 || t2 < 0 || t3 < 0) break;
+        if (t4 < 0 || t5 < 0) continue;
+        double l = lpref - lfact(k) - lfact(t1) - lfact(t2) - lfact(t3) - lfact(t4) - lfact(t5);
+        sum += ((k & 1) ? -1.0 : 1.0) * std::exp(l);
+    }
+    return sum;
+}
+
+// C2 table, src/RooDalitzAmplitude.cxx:96-107: CG(1,1;L,S,0,1) for 2S = 1,3,..,11
+// and 2L = S-1, S+1, i.e. <L/2 0; S/2 1/2 | 1/2 1/2>.
+double C2(int iS, int iL) {
+    const int S = 2 * iS + 1;
+    const int L = S - 1 + 2 * iL;
+    return cg(1, 1, L, S, 0, 1);
+}
+
+// --------------------------------------------------- model bookkeeping
+struct Model {
+    const pwa_model_desc* d;
+    int nR, nZ, nX, n;
+    int chain(int r) const { return r < nR ? 0 : (r < nR + nZ ? 1 : 2); }
+};
+
+// setup_paraList L rules, src/RooDalitzAmplitude.cxx:3068-3139
+int orbital_L(const Model& M, int r) {
+    const int S2 = M.d->spin2[r];
+    const int P = M.d->parity[r];
+    const float S = S2 / 2.0f;
+    int LR;
+    const int ch = M.chain(r);
+    if (ch == 0) {
+        LR = (int)(S - 0.5f);
+        if (std::pow(-1.0, LR + 1) * P < 0) LR += 1;
+    } else if (ch == 1) {
+        LR = std::max((int)(S - 1.5f), 0);
+        if (std::pow(-1.0, LR + 1) * P < 0) LR += 1;
+    } else {
+        LR = std::max(std::abs((int)(S - 1)), 0);
+        if (std::pow(-1.0, LR) * P < 0) LR += 1;
+    }
+    return LR;
+}
+
+int param_offset(const Model& M, int r) {
+    if (r < M.nR) return r * PWA_NPAR_LAMBDA;
+    if (r < M.nR + M.nZ) return M.nR * PWA_NPAR_LAMBDA + (r - M.nR) * PWA_NPAR_PC;
+    return M.nR * PWA_NPAR_LAMBDA + M.nZ * PWA_NPAR_PC + (r - M.nR - M.nZ) * PWA_NPAR_ZCS;
+}
+int param_size(const Model& M, int r) {
+    const int ch = M.chain(r);
+    return ch == 0 ? PWA_NPAR_LAMBDA : (ch == 1 ? PWA_NPAR_PC : PWA_NPAR_ZCS);
+}
+
+const int Hel1Two[6] = {1, 1, -1, -1, 3, -3}; // src/RooDalitzAmplitude.cxx:111
+const int Hel2[6] = {1, 0, 0, -1, 1, -1};     // src/RooDalitzAmplitude.cxx:112
+
+// PartWaveCoff, src/RooDalitzAmplitude.cxx:5532-5596 (all arguments 2x)
+cd PartWaveCoff(const double* ires, int J, int lambda2, int lambda3) {
+    int ib = 0;
+    cd c(0);
+    const int s2 = J, s3 = 2, JB = 1;
+    const int m1 = lambda2 - lambda3;
+    for (int S = std::abs(J - 2); S <= J + 2; S += 2) {
+        for (int L = S - 1; L <= S + 1; L += 2) {
+            cd Mc(ires[ib * 2], ires[ib * 2 + 1]);
+            if (std::norm(Mc) < 1e-10) { ib++; continue; }
+            double fkw = (L + 1.0) / (JB + 1.0);
+            double ccg = cg(S, m1, s2, s3, lambda2, -lambda3);
+            cd tmp = std::sqrt(fkw) * ccg * C2((S - 1) / 2, (L - S + 1) / 2) * Mc;
+            if (m1 < 0 && L > S) tmp = -tmp;
+            c += tmp;
+            ib++;
+        }
+    }
+    return c;
+}
+
+// PartWaveCoffZ, src/RooDalitzAmplitude.cxx:5599-5720
+cd PartWaveCoffZ(const double* ires, int size, int J, int lambdaZ, int lambdaPsi, int lambdap) {
+    cd HB(0, 0);
+    const int SB = J;
+    for (int LB = SB - 1; LB <= SB + 1; LB += 2) {
+        cd tmp = cd(1.0, 0) * C2((SB - 1) / 2, (LB - SB + 1) / 2) * std::sqrt((LB + 1.0) / 2.0);
+        if (LB > SB) {
+            cd Mc(ires[0], ires[1]);
+            if (std::norm(Mc) < 1e-10) continue;
+            tmp = tmp * Mc;
+            if (lambdaZ < 0) tmp = -tmp;
+        }
+        HB += tmp;
+    }
+    const int parity = (int)ires[size - 1];
+    int ib = 1;
+    cd c(0);
+    const int m1 = lambdaPsi - lambdap;
+    for (int S = 1; S <= 3; S += 2) {
+        for (int L = std::abs(J - S); L <= J + S; L += 2) {
+            if ((std::pow(-1.0, (int)(L / 2) + 1) * parity) < 0) continue;
+            if (S < std::abs(m1)) { ib++; continue; }
+            double ccg1 = cg(S, m1, 2, 1, lambdaPsi, -lambdap);
+            cd Mc(ires[ib * 2], ires[ib * 2 + 1]);
+            if (std::norm(Mc) < 1e-10) { ib++; continue; }
+            double fkw = (L + 1.0) / (J + 1.0);
+            double ccg;
+            if (J == 1) {
+                ccg = C2((S - 1) / 2, (L - S + 1) / 2);
+                if (m1 < 0 && L > S) ccg = -ccg;
+            } else {
+                ccg = cg(J, m1, L, S, 0, m1);
+            }
+            c += std::sqrt(fkw) * ccg * ccg1 * Mc;
+            ib++;
+        }
+    }
+    return c * HB;
+}
+
+// PartWaveCoffX, src/RooDalitzAmplitude.cxx:5724-5828
+cd PartWaveCoffX(const double* ires, int size, int J, int lambdaZ, int lambdaPsi, int lambdap) {
+    int ib = 4;
+    cd HZ(0, 0);
+    const int parity = (int)ires[size - 1];
+    const int SZ = 2;
+    bool first = true;
+    for (int LZ = J - SZ; LZ <= SZ + J; LZ += 2) {
+        if (J == 0 && LZ != SZ) continue;
+        if (parity * std::pow(-1.0, (int)(LZ / 2)) < 0) continue;
+        double tmp1 = (LZ + 1.0) / (J + 1.0);
+        double cg2 = cg(J, lambdaPsi, LZ, SZ, 0, lambdaPsi);
+        if (first) {
+            first = false;
+            HZ += cd(1.0, 0) * std::sqrt(tmp1) * cg2;
+        } else {
+            cd Mc(ires[2 * ib], ires[2 * ib + 1]);
+            if (std::norm(Mc) < 1e-10) { ib++; continue; }
+            HZ += Mc * std::sqrt(tmp1) * cg2;
+        }
+    }
+    ib = 0;
+    cd c(0);
+    const int m1 = lambdaZ - lambdap;
+    for (int S = J - 1; S <= J + 1; S += 2) {
+        if (S < 0) continue;
+        for (int L = S - 1; L <= S + 1; L += 2) {
+            double ccg1 = C2((S - 1) / 2, (L - S + 1) / 2);
+            if (m1 < 0 && L > S) ccg1 = -ccg1;
+            cd Mc(ires[ib * 2], ires[ib * 2 + 1]);
+            if (std::norm(Mc) < 1e-10) { ib++; continue; }
+            double fkw = (L + 1.0) / 2.0;
+            double ccg = cg(S, m1, J, 1, lambdaZ, -lambdap);
+            c += std::sqrt(fkw) * ccg * ccg1 * Mc;
+            ib++;
+        }
+    }
+    return c * HZ;
+}
+
+// Amplitude-level parameters built from the fit parameters: the content of
+// the reference's h_paraList (cu_init_data, src/RooDalitzAmplitude.cxx:3141-3210)
+struct AmpParams {
+    std::vector<double> m0, g0;          // per resonance
+    std::vector<int> L;                  // per resonance
+    std::vector<cd> vamp;                // helicity couplings, reference order
+    std::vector<int> ampcount;           // first vamp index per resonance
+    double km[PWA_KM_NCLASS][2 * PWA_KM_NPOLE * PWA_KM_NCH]; // g[4][4] then f[4][4] per class
+    std::vector<int> ikmcount;           // per resonance (Pc KMATRIX members), else -1
+};
+
+int km_class(int S2, int LR) { return (LR > 0) ? ((S2 == 3) ? 3 : 2) : ((S2 == 3) ? 1 : 0); }
+
+// ConfigureCouplings<true>, src/RooDalitzAmplitude.cxx:2669-2986, with the
+// Misha sign factors (#define Misha, :10) and PAW (#define PAW, :79).
+void build_amp(const Model& M, const double* par, AmpParams& ap) {
+    const int n = M.n;
+    ap.m0.assign(n, 0);
+    ap.g0.assign(n, 0);
+    ap.L.assign(n, 0);
+    ap.vamp.clear();
+    ap.ampcount.assign(n, 0);
+    ap.ikmcount.assign(n, -1);
+    std::memset(ap.km, 0, sizeof(ap.km));
+    std::vector<double> galpha[4], fscat[4];
+    std::vector<std::vector<double>> fextra[4]; // PWA_F_KM_FULL_F: f[i][i..3] of NR member i
+    for (int r = 0; r < n; ++r) {
+        const double* ires = par + param_offset(M, r);
+        const int size = param_size(M, r);
+        ap.m0[r] = ires[size - 5];
+        ap.g0[r] = ires[size - 4];
+        ap.L[r] = orbital_L(M, r);
+        const int S2 = M.d->spin2[r];
+        const float S = S2 / 2.0f;
+        const int parity = M.d->parity[r];
+        ap.ampcount[r] = (int)ap.vamp.size();
+        const int ch = M.chain(r);
+        if (ch == 0) {
+            for (int k = 0; k <= 1; ++k) {
+                double paritypart = (k == 0) ? std::pow(-1., (int)(S - 1.5f)) * parity : 1.;
+                for (int m = 0; m < 6; ++m) {
+                    if (S2 == 1 && m >= 4) continue;
+                    int hel2 = Hel2[m];
+                    cd a = PartWaveCoff(ires, S2, Hel1Two[m], hel2 * 2) * paritypart;
+                    a = a * std::pow(-1.0, (1 - hel2));
+                    a = a * cd(ires[size - 7], ires[size - 6]);
+                    ap.vamp.push_back(a);
+                }
+            }
+        } else if (ch == 1) {
+            for (int k = 0; k <= 1; ++k) {
+                for (int helpsi = -1; helpsi <= 1; ++helpsi)
+                    for (int ihz = 0; ihz <= 1; ++ihz) {
+                        float helZ = (ihz == 0) ? -0.5f : 0.5f;
+                        for (int ihp = 0; ihp <= 1; ++ihp) {
+                            float help = (ihp == 0) ? -0.5f : 0.5f;
+                            if (std::fabs((float)helpsi - help) > S) continue;
+                            cd a = PartWaveCoffZ(ires, size, S2, (int)(helZ * 2.), (int)(helpsi * 2.), (int)(help * 2.));
+                            a = a * std::pow(-1.0, (1 - ihp));
+                            ap.vamp.push_back(a);
+                        }
+                    }
+            }
+            if (M.d->type[r] == PWA_KMATRIX) {
+                int ik = (S2 == 3) ? 1 : 0;
+                if (parity > 0) ik += 2;
+                if (ap.m0[r] > 0) {
+                    for (int ib = 8; ib < 12; ++ib) galpha[ik].push_back(ires[ib]);
+                } else {
+                    const int i = (int)fscat[ik].size();
+                    fscat[ik].push_back(ires[8]);
+                    std::vector<double> row;
+                    if (i >= 1)
+                        for (int j = i; j < PWA_KM_NCH; ++j) row.push_back(ires[9 + (j - i)]);
+                    fextra[ik].push_back(row);
+                }
+            }
+        } else {
+            for (int k = 0; k <= 1; ++k) {
+                for (int helpsi = -1; helpsi <= 1; ++helpsi) {
+                    if (std::abs(helpsi) > S) continue;
+                    for (int ihp = 0; ihp <= 1; ++ihp) {
+                        float help = (ihp == 0) ? -0.5f : 0.5f;
+                        for (int ihz = 0; ihz <= 1; ++ihz) {
+                            float helZ = (ihz == 0) ? help - 0.5f : help + 0.5f;
+                            if (std::fabs(helZ) > S) continue;
+                            cd a = PartWaveCoffX(ires, size, S2, (int)(helZ * 2.), (int)(helpsi * 2.), (int)(help * 2.));
+                            a = a * std::pow(-1.0, (1 - ihp)) * std::pow(-1.0, (1 - helpsi));
+                            ap.vamp.push_back(a);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // f_scat row 0 mirrored into column 0 (:2902-2916), then the KM block
+    // layout of cu_init_data (:3175-3190): per class g[Npmax*NKMC], f[NKMC*NKMC].
+    for (int ik = 0; ik < 4; ++ik) {
+        fscat[ik].resize(PWA_KM_NCH * PWA_KM_NCH, 0.0);
+        for (int i = 1; i < PWA_KM_NCH; ++i) fscat[ik][PWA_KM_NCH * i] = fscat[ik][i];
+        if (M.d->flags & PWA_F_KM_FULL_F) {
+            for (size_t i = 1; i < fextra[ik].size() && i < (size_t)PWA_KM_NCH; ++i)
+                for (size_t jj = 0; jj < fextra[ik][i].size(); ++jj) {
+                    const int j = (int)i + (int)jj;
+                    fscat[ik][PWA_KM_NCH * i + j] = fextra[ik][i][jj];
+                    fscat[ik][PWA_KM_NCH * j + i] = fextra[ik][i][jj];
+                }
+        }
+        for (size_t i = 0; i < galpha[ik].size() && i < 16; ++i) ap.km[ik][i] = galpha[ik][i];
+        for (int i = 0; i < 16; ++i) ap.km[ik][16 + i] = fscat[ik][i];
+    }
+    // _constant_Pc_ikmcount, src/kernel_calEva.cu:978-1010
+    int cnt[4] = {0, 0, 0, 0};
+    for (int r = M.nR; r < M.nR + M.nZ; ++r) {
+        if (M.d->type[r] == PWA_KMATRIX) {
+            int ik = km_class(M.d->spin2[r], ap.L[r]);
+            ap.ikmcount[r] = cnt[ik]++;
+        }
+    }
+}
+
+// ------------------------------------------------------------ lineshapes
+struct Phys {
+    double m_lb, m_k, m_p, m_jpsi, ffr, ffb;
+};
+
+// getp, src/cu_DPFPropogator.cu:33-36
+double getp(double Mm, double m1, double m2) {
+    double M2 = Mm * Mm;
+    return std::sqrt(std::fabs((M2 - (m1 + m2) * (m1 + m2)) * (M2 - (m1 - m2) * (m1 - m2)))) / 2. / Mm;
+}
+
+// F_BW, src/cu_DPFPropogator.cu:69-97 (Blatt-Weisskopf ratio, L <= 5)
+double F_BW(double r, int L, double p, double p0) {
+    double z = (r * p) * (r * p);
+    double z0 = (r * p0) * (r * p0);
+    double c = 1.0;
+    switch (L) {
+    case 1: c = std::sqrt((1. + z0) / (1. + z)); break;
+    case 2: c = std::sqrt((z0 * z0 + 3. * z0 + 9.) / (z * z + 3. * z + 9.)); break;
+    case 3:
+        c = std::sqrt((z0 * z0 * z0 + 6 * z0 * z0 + 45 * z0 + 225) / (z * z * z + 6 * z * z + 45 * z + 225));
+        break;
+    case 4:
+        c = std::sqrt((std::pow(z0, 4) + 10 * std::pow(z0, 3) + 135 * z0 * z0 + 1575 * z0 + 11025) /
+                      (std::pow(z, 4) + 10 * std::pow(z, 3) + 135 * z * z + 1575 * z + 11025));
+        break;
+    case 5:
+        c = std::sqrt((std::pow(z0, 5) + 15 * std::pow(z0, 4) + 315 * std::pow(z0, 3) + 6300 * std::pow(z0, 2) + 99225 * z0 + 893025) /
+                      (std::pow(z, 5) + 15 * std::pow(z, 4) + 315 * std::pow(z, 3) + 6300 * std::pow(z, 2) + 99225 * z + 893025));
+        break;
+    }
+    return c;
+}
+
+// F_BW2, src/cu_DPFPropogator.cu:99-127
+double F_BW2(double r, int L, double p2, double p20) {
+    double z = r * r * p2;
+    double z0 = r * r * std::fabs(p20);
+    double c = 1.0;
+    switch (L) {
+    case 1: c = (1. + z0) / (1. + z); break;
+    case 2: c = (z0 * z0 + 3 * z0 + 9.) / (z * z + 3 * z + 9.); break;
+    case 3: c = (z0 * z0 * z0 + 6 * z0 * z0 + 45 * z0 + 225) / (z * z * z + 6 * z * z + 45 * z + 225); break;
+    case 4:
+        c = (std::pow(z0, 4) + 10 * std::pow(z0, 3) + 135 * z0 * z0 + 1575 * z0 + 11025) /
+            (std::pow(z, 4) + 10 * std::pow(z, 3) + 135 * z * z + 1575 * z + 11025);
+        break;
+    case 5:
+        c = (std::pow(z0, 5) + 15 * std::pow(z0, 4) + 315 * std::pow(z0, 3) + 6300 * std::pow(z0, 2) + 99225 * z0 + 893025) /
+            (std::pow(z, 5) + 15 * std::pow(z, 4) + 315 * std::pow(z, 3) + 6300 * std::pow(z, 2) + 99225 * z + 893025);
+        break;
+    }
+    return c;
+}
+
+// getFBW2, src/cu_DPFPropogator.cu:129-135
+double getFBW2(double M2, double m1, double m2, int L, double ffb) {
+    if (L == 0) return 1.0;
+    double p2 = (M2 - (m1 + m2) * (m1 + m2)) * (M2 - (m1 - m2) * (m1 - m2)) / M2 / 4.0;
+    double M20 = 25.0;
+    double p20 = (M20 - (m1 + m2) * (m1 + m2)) * (M20 - (m1 - m2) * (m1 - m2)) / M20 / 4.0;
+    return std::pow(p2 / p20, L) * F_BW2(ffb, L, p2, p20);
+}
+
+// Complex square root with the reference's branch rule (the cuCsqrtf
+// extension, RooTDalitz/cuComplex.h:637-653): 0 inside |x|,|y| < 1e-6; the
+// imaginary part is negative only if Im < 0 and |Im/Re| > 1e-3.
+cd csqrt_ref(cd x) {
+    if (std::fabs(x.real()) < 0.000001 && std::fabs(x.imag()) < 0.000001) return cd(0, 0);
+    double radius = std::abs(x);
+    double cosA = x.real() / radius;
+    double re = std::sqrt(radius * (cosA + 1.0) / 2.0);
+    double im = std::sqrt(radius * (1.0 - cosA) / 2.0);
+    if (std::signbit(x.imag()) && std::fabs(x.imag() / x.real()) > 0.001) im = -im;
+    return cd(re, im);
+}
+
+// getrho, src/cu_DPFPropogator.cu:38-48
+cd getrho(double M2, double m1, double m2, int L, double ffb) {
+    double rho2 = (M2 - (m1 + m2) * (m1 + m2)) * (M2 - (m1 - m2) * (m1 - m2));
+    double rho = std::sqrt(std::fabs(rho2)) / M2 * getFBW2(M2, m1, m2, L, ffb);
+    if (rho2 >= 0) return cd(rho, 0);
+    return cd(0, rho);
+}
+
+// getrho_complexMass, src/cu_DPFPropogator.cu:50-67 (its barrier factor
+// getFBW2_complexMass is identically 1, :137-139)
+cd getrho_cm(double M2, cd m1, cd m2) {
+    cd part1 = (m1 + m2) * (m1 + m2) - M2;
+    cd part2 = (m1 - m2) * (m1 - m2) - M2;
+    return csqrt_ref(part1 * part2) * (1.0 / M2);
+}
+
+// BTerm (Lambda* chain), src/cu_DPFPropogator.cu:155-181 (LINESHAPE_LBPLUSONE false)
+double BTerm(const Phys& P, double J, double mpp, double mr) {
+    int LB = (J <= 1) ? 0 : (int)(J - 1.5);
+    double pB = getp(P.m_lb, P.m_jpsi, mpp);
+    double pB0 = pB;
+    if (mr < -0.) {
+        double mmin = P.m_k + P.m_p, mmax = P.m_lb - P.m_jpsi;
+        pB0 = getp(P.m_lb, P.m_jpsi, (mmin + mmax) / 2.);
+    }
+    if (mr > 1e-10 && mr + P.m_jpsi < P.m_lb) pB0 = getp(P.m_lb, P.m_jpsi, mr);
+    if (mr + P.m_jpsi > P.m_lb) {
+        double mmin = P.m_k + P.m_p, mmax = P.m_lb - P.m_jpsi;
+        double mreff = mmin + 0.5 * (mmax - mmin) * (1.0 + std::tanh((mr - (mmin + mmax) / 2.) / (mmax - mmin)));
+        pB0 = getp(P.m_lb, P.m_jpsi, mreff);
+    }
+    return F_BW(P.ffb, LB, pB, pB0) * std::pow(pB / P.m_lb, LB) * std::pow(P.m_lb / pB0, LB);
+}
+
+// BTermZ (Pc chain), src/cu_DPFPropogator.cu:183-203
+double BTermZ(const Phys& P, double J, double mpp, double mr) {
+    int LB = (int)(J - 0.5);
+    double pB = getp(P.m_lb, P.m_k, mpp);
+    double pB0 = pB;
+    if (mr < -0.) {
+        double mmin = P.m_jpsi + P.m_p, mmax = P.m_lb - P.m_k;
+        pB0 = getp(P.m_lb, P.m_k, (mmin + mmax) / 2.);
+    }
+    if (mr > 1e-10 && mr + P.m_k < P.m_lb) pB0 = getp(P.m_lb, P.m_k, mr);
+    return F_BW(P.ffb, LB, pB, pB0) * std::pow(pB / P.m_lb, LB);
+}
+
+// BTermX (Zcs chain), src/cu_DPFPropogator.cu:205-217
+double BTermX(const Phys& P, double J, double mpp, double mr) {
+    int LB = (int)(J - 0.5);
+    double pB = getp(P.m_lb, P.m_p, mpp);
+    double pB0 = pB;
+    if (mr < -0) return std::pow(pB / P.m_lb, LB);
+    if (mr > 1e-10 && mr + P.m_p < P.m_lb) pB0 = getp(P.m_lb, P.m_p, mr);
+    return F_BW(P.ffb, LB, pB, pB0) * std::pow(pB / P.m_lb, LB);
+}
+
+// breit_wigner, src/cu_DPFPropogator.cu:219-224: 1/(m^2 - s - i m Gamma)
+cd breit_wigner(double m, double gamma, double e2) {
+    double de2 = m * m - e2;
+    double gamma_m = gamma * m;
+    double d = de2 * de2 + gamma_m * gamma_m;
+    return cd(de2 / d, gamma_m / d);
+}
+
+double bw_norm(double m_r, double gamma_r) {
+    double kr = std::sqrt(m_r * m_r + gamma_r * gamma_r);
+    return 0.948850 * m_r * std::sqrt(gamma_r * kr / std::sqrt(m_r * (m_r + kr)));
+}
+
+// BW_AMP (Lambda*, K- p), src/cu_DPFPropogator.cu:248-314
+cd BW_AMP(const Phys& P, double m_r, double gamma_r, double mpp, int LR) {
+    double m2pp = mpp * mpp;
+    double pR = getp(mpp, P.m_p, P.m_k);
+    double pR0 = pR;
+    if (m_r < -0) {
+        double mmin = P.m_k + P.m_p, mmax = P.m_lb - P.m_jpsi;
+        pR0 = getp((mmin + mmax) / 2., P.m_p, P.m_k);
+    }
+    if (m_r > P.m_p + P.m_k) pR0 = getp(m_r, P.m_p, P.m_k);
+    if (m_r <= P.m_p + P.m_k && m_r > 1e-10) {
+        double mmin = P.m_k + P.m_p, mmax = P.m_lb - P.m_jpsi;
+        double mreff = mmin + 0.5 * (mmax - mmin) * (1.0 + std::tanh((m_r - (mmin + mmax) / 2.) / (mmax - mmin)));
+        pR0 = getp(mreff, P.m_p, P.m_k);
+    }
+    double FR = F_BW(P.ffr, LR, pR, pR0);
+    if (m_r < -0) return cd(std::pow(pR / pR0, LR), 0);
+    double gamma = gamma_r;
+    if (m_r > P.m_p + P.m_k) {
+        gamma = gamma_r * std::pow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    } else if (m_r < 1.43 && m_r > 0) {
+        const double msig = 1.18937, mpi0 = 0.13957018;
+        double pRSigPi = getp(mpp, msig, mpi0);
+        gamma = 0.0;
+        pR0 = getp(m_r, msig, mpi0);
+        FR = 1.0;
+        gamma += gamma_r * std::pow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+        FR = 1.0;
+        gamma += 0.0505 * std::pow(pRSigPi / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    }
+    double factor = FR * std::pow(pR / pR0, LR) * bw_norm(m_r, gamma_r);
+    return factor * breit_wigner(m_r, gamma, m2pp);
+}
+
+// BW_AMPZ (Pc, J/psi p) and BW_AMPX (Zcs, J/psi K), src/cu_DPFPropogator.cu:316-336, :226-246
+cd BW_AMP2(double ma, double mb, double ffr, double m_r, double gamma_r, double mpp, int LR) {
+    double m2pp = mpp * mpp;
+    double pR = getp(mpp, ma, mb);
+    double pR0 = pR;
+    if (m_r > ma + mb) pR0 = getp(m_r, ma, mb);
+    double FR = F_BW(ffr, LR, pR, pR0);
+    double gamma = gamma_r;
+    if (m_r > ma + mb) gamma = gamma_r * std::pow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    double factor = FR * std::pow(pR / pR0, LR) * bw_norm(m_r, gamma_r);
+    return factor * breit_wigner(m_r, gamma, m2pp);
+}
+
+// Row 0 of S^-1 by the reference's transposed LU without pivoting
+// (solve_NxN<double2,double2,0,N>, RooTDalitz/cuSolve4x4.h:278-350).
+void solve_row0(const cd* S, int n, cd* x) {
+    cd A[4][4], L[4][4], U[4][4], invU[4];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[i][j] = S[j * n + i];
+    for (int i = 0; i < n; ++i) {
+        for (int col = i; col < n; ++col) {
+            U[i][col] = A[i][col];
+            for (int k = 0; k < i; ++k) U[i][col] -= L[i][k] * U[k][col];
+        }
+        invU[i] = 1.0 / U[i][i];
+        for (int row = i + 1; row < n; ++row) {
+            L[row][i] = A[row][i];
+            for (int k = 0; k < i; ++k) L[row][i] -= L[row][k] * U[k][i];
+            L[row][i] = L[row][i] * invU[i];
+        }
+    }
+    cd y[4];
+    for (int i = 0; i < n; ++i) {
+        y[i] = (i == 0) ? cd(1, 0) : cd(0, 0);
+        for (int j = 0; j < i; ++j) y[i] -= L[i][j] * y[j];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        x[i] = y[i];
+        for (int j = i + 1; j < n; ++j) x[i] -= U[i][j] * x[j];
+        x[i] = invU[i] * x[i];
+    }
+}
+
+// Default_KMatrix, src/cu_DPFPropogator.cu:588-813, with the channel masses
+// and threshold-shift channels taken from the model description instead of
+// the compile-time constants (src/cu_DPFPropogator.cu:591-596, 609-672, 693-718).
+void KMatrix(const Model& M, const AmpParams& ap, const Phys& P, double mpp, int J_r, int P_r, int ik, cd* Mtr,
+             bool clamp) {
+    const pwa_model_desc* d = M.d;
+    int Lna = (P_r == 1) ? 1 : 0;
+    double s = mpp * mpp;
+    cd rho[PWA_KM_NCH];
+    for (int c = 0; c < PWA_KM_NCH; ++c) {
+        if (d->km_cplx[ik][c])
+            rho[c] = getrho_cm(s, cd(d->km_ma[ik][c], 0.0), cd(d->km_mb[ik][c], -d->km_hw[ik][c]));
+        else
+            rho[c] = getrho(s, d->km_ma[ik][c], d->km_mb[ik][c], Lna, P.ffb);
+    }
+    const double* g_alpha = &ap.km[ik][0];
+    const double* f_scat = &ap.km[ik][16];
+    int Npole = 0;
+    double ms[PWA_KM_NPOLE + 1] = {1, 1, 1, 1, 1};
+    int ikm = 1;
+    for (int ir = M.nR; ir < M.nR + M.nZ; ++ir) {
+        double mfit = ap.m0[ir];
+        int iChannel = (int)ap.g0[ir];
+        int S = d->spin2[ir];
+        int type = d->type[ir];
+        int parity = d->parity[ir];
+        if (type == PWA_KMATRIX && S == J_r && parity == P_r) {
+            if (mfit > 0) {
+                double m0 = mfit;
+                if (iChannel > 0) {
+                    double mfit2 = mfit * mfit;
+                    cd rhoC = getrho_cm(mfit2, cd(d->km_shift_ma[iChannel], 0.0),
+                                        cd(d->km_shift_mb[iChannel], -d->km_shift_hw[iChannel]));
+                    double gC = g_alpha[Npole * PWA_KM_NCH + iChannel];
+                    cd nt = cd(0.0, gC * gC) * rhoC;
+                    m0 = std::sqrt(mfit2 + nt.real());
+                }
+                double denorm = m0 * m0 - s;
+                if (clamp && std::fabs(denorm) < 1e-3) denorm = 1e-3;
+                Npole++;
+                ms[0] *= denorm;
+                for (int ip = 1; ip <= PWA_KM_NPOLE; ++ip) {
+                    if (ip == ikm) continue;
+                    ms[ip] *= denorm;
+                }
+                ikm++;
+            }
+        }
+    }
+    double mg[PWA_KM_NPOLE][PWA_KM_NCH];
+    int k2 = 0;
+    for (int i = 0; i < Npole; ++i)
+        for (int j = 0; j < PWA_KM_NCH; ++j) mg[i][j] = g_alpha[k2++] * ms[i + 1];
+    double fs[4][4], kma[4][4];
+    k2 = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) fs[i][j] = f_scat[k2++] * ms[0];
+    for (int i = 0; i < 4; ++i)
+        for (int j = i; j < 4; ++j) {
+            kma[i][j] = fs[i][j];
+            for (int k = 0; k < Npole; ++k) kma[i][j] += g_alpha[k * PWA_KM_NCH + i] * mg[k][j];
+            kma[j][i] = kma[i][j];
+        }
+    cd KK[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            KK[i * 4 + j] = cd(0, -1.0) * (kma[i][j] * rho[j]);
+            if (i == j) KK[i * 4 + j] += ms[0];
+        }
+    cd T[4];
+    solve_row0(KK, 4, T);
+    double na = 1.0;
+    if (Lna >= 1) na = getFBW2(s, d->km_ma[ik][0], d->km_mb[ik][0], Lna, P.ffb);
+    for (int i = 0; i < PWA_KM_NCH + Npole; ++i) Mtr[i] = 0;
+    for (int ip = 0; ip < Npole; ++ip) {
+        for (int i = 0; i < 4; ++i) Mtr[ip] += T[i] * mg[ip][i];
+        Mtr[ip] *= na;
+    }
+    for (int i = 0; i < 4; ++i) Mtr[Npole + i] = T[i] * (ms[0] * na);
+}
+
+// ------------------------------------------------ per-event angular tables
+// struct cu_Jpsihh_dlz (RooTDalitz/cu_Jpsihh_dlz.h:9-25), filled in FP64.
+struct Dlz {
+    double mpk, mjpsip, mjpsik;
+    cd A[2][2][6];
+    double DL[5][2][6];
+    cd AZ[2][2][12];
+    double DZ[4][2][12];
+    cd AX[2][2][12];
+    double DX[4][2][12];
+};
+
+// MEangPart / MEangZPart / MEangXPart, src/RooDalitzAmplitude.cxx:2593-2666
+cd polar(double phi) { return cd(std::cos(phi), std::sin(phi)); }
+
+// filldlz, src/RooDalitzAmplitude.cxx:7086-7307
+void filldlz(Dlz& z, const double* ev /* PWA_COL_* values */) {
+    z.mpk = ev[PWA_COL_MKP];
+    z.mjpsip = ev[PWA_COL_MJPSIP];
+    z.mjpsik = ev[PWA_COL_MJPSIK];
+    double theta = std::acos(ev[PWA_COL_COS_LB]);
+    double theta1 = std::acos(ev[PWA_COL_COS_L]);
+    double theta2 = std::acos(ev[PWA_COL_COS_JPSI]);
+    double thetaB = std::acos(ev[PWA_COL_Z_COS_LB]);
+    double thetaZ = std::acos(ev[PWA_COL_Z_COS_Z]);
+    double thetaPsi = std::acos(ev[PWA_COL_Z_COS_JPSI]);
+    double thetap = std::acos(ev[PWA_COL_COS_P]);
+    double thetaB_X = std::acos(ev[PWA_COL_X_COS_LB]);
+    double thetaX_X = std::acos(ev[PWA_COL_X_COS_X]);
+    double thetaPsi_X = std::acos(ev[PWA_COL_X_COS_JPSI]);
+    double thetap_X = std::acos(ev[PWA_COL_X_COS_P]);
+    const double phi1 = ev[PWA_COL_PHI_K], phi2 = ev[PWA_COL_PHI_MU];
+    const double phiZ = ev[PWA_COL_Z_PHI_Z], phiPsi = ev[PWA_COL_Z_PHI_JPSI], phiMu = ev[PWA_COL_Z_PHI_MU];
+    const double alphaMu = ev[PWA_COL_ALPHA_MU];
+    const double xphiX = ev[PWA_COL_X_PHI_X], xphiPsi = ev[PWA_COL_X_PHI_JPSI], xphiMu = ev[PWA_COL_X_PHI_MU];
+    const double xalphaMu = ev[PWA_COL_X_ALPHA_MU];
+
+    for (int i = 0; i <= 1; ++i) {
+        int delhelmu = (i == 0) ? -1 : 1;
+        for (int j = 0; j <= 1; ++j) {
+            double hel0 = (j == 0) ? -0.5 : 0.5;
+            for (int m = 0; m < 6; ++m) {
+                double hel1 = Hel1Two[m] / 2.;
+                int hel2 = Hel2[m];
+                double partphi = hel1 * phi1 + hel2 * phi2;
+                z.A[i][j][m] = polar(partphi) * dfunhalf(theta, (int)(hel0 * 2.), (int)((hel1 - hel2) * 2.)) *
+                               dfunone(theta2, hel2, delhelmu);
+            }
+        }
+    }
+    for (int s = 0; s < 5; ++s) {
+        double S = (2 * s + 1) / 2.;
+        for (int k = 0; k <= 1; ++k) {
+            double hel3 = (k == 0) ? -0.5 : 0.5;
+            for (int m = 0; m < 6; ++m) {
+                if (std::fabs(S - 0.5) < 1e-10 && m >= 4) { z.DL[s][k][m] = 0; continue; }
+                z.DL[s][k][m] = dfun(theta1, S, Hel1Two[m] / 2., hel3);
+            }
+        }
+    }
+    for (int i = 0; i <= 1; ++i) {
+        int delhelmu = (i == 0) ? -1 : 1;
+        for (int j = 0; j <= 1; ++j) {
+            double hel0 = (j == 0) ? -0.5 : 0.5;
+            int m = 0;
+            for (int helpsi = -1; helpsi <= 1; ++helpsi)
+                for (int ihz = 0; ihz <= 1; ++ihz) {
+                    double helZ = (ihz == 0) ? -0.5 : 0.5;
+                    for (int ihp = 0; ihp <= 1; ++ihp) {
+                        double partphi = hel0 * phiZ + helpsi * phiMu + helZ * phiPsi + delhelmu * alphaMu;
+                        z.AZ[i][j][m] = polar(partphi) * dfunone(thetaPsi, helpsi, delhelmu) *
+                                        dfunhalf(thetaB, (int)(hel0 * 2.), (int)(helZ * 2.));
+                        m++;
+                    }
+                }
+        }
+    }
+    for (int s = 0; s < 4; ++s) {
+        double S = (2 * s + 1) / 2.;
+        for (int k = 0; k <= 1; ++k) {
+            double hel3 = (k == 0) ? -0.5 : 0.5;
+            int m = 0;
+            for (int helpsi = -1; helpsi <= 1; ++helpsi)
+                for (int ihz = 0; ihz <= 1; ++ihz) {
+                    double helZ = (ihz == 0) ? -0.5 : 0.5;
+                    for (int ihp = 0; ihp <= 1; ++ihp) {
+                        double help = (ihp == 0) ? -0.5 : 0.5;
+                        if (std::fabs(helpsi - help) > S) { z.DZ[s][k][m] = 0; m++; continue; }
+                        z.DZ[s][k][m] = dfunhalf(thetap, (int)(help * 2.), (int)(hel3 * 2.)) *
+                                        dfun(thetaZ, S, helZ, helpsi - help);
+                        m++;
+                    }
+                }
+        }
+    }
+    for (int i = 0; i <= 1; ++i) {
+        int delhelmu = (i == 0) ? -1 : 1;
+        for (int j = 0; j <= 1; ++j) {
+            double hel0 = (j == 0) ? -0.5 : 0.5;
+            int m = 0;
+            for (int helpsi = -1; helpsi <= 1; ++helpsi)
+                for (int ihp = 0; ihp <= 1; ++ihp) {
+                    double help = (ihp == 0) ? -0.5 : 0.5;
+                    for (int ihz = 0; ihz <= 1; ++ihz) {
+                        double helZ = (ihz == 0) ? help - 0.5 : help + 0.5;
+                        double partphi = hel0 * xphiX + helpsi * xphiMu + (helZ - help) * xphiPsi + delhelmu * xalphaMu;
+                        z.AX[i][j][m] = polar(partphi) * dfunone(thetaPsi_X, helpsi, delhelmu) *
+                                        dfunhalf(thetaB_X, (int)(hel0 * 2.), (int)((helZ - help) * 2.));
+                        m++;
+                    }
+                }
+        }
+    }
+    std::memset(z.DX, 0, sizeof(z.DX));
+    for (int s = 0; s < 4; ++s) {
+        double S = s;
+        for (int k = 0; k <= 1; ++k) {
+            double hel3 = (k == 0) ? -0.5 : 0.5;
+            int m = 0;
+            for (int helpsi = -1; helpsi <= 1; ++helpsi) {
+                if (std::abs(helpsi) > S) continue;
+                for (int ihp = 0; ihp <= 1; ++ihp) {
+                    double help = (ihp == 0) ? -0.5 : 0.5;
+                    for (int ihz = 0; ihz <= 1; ++ihz) {
+                        double helZ = (ihz == 0) ? help - 0.5 : help + 0.5;
+                        if (std::fabs(helZ) > S) continue;
+                        z.DX[s][k][m] = dfunhalf(thetap_X, (int)(help * 2.), (int)(hel3 * 2.)) *
+                                        dfun(thetaX_X, S, helZ, (double)(-1 * helpsi));
+                        m++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------- amplitudes
+// GetAmpLz, src/kernel_calEva.cu:85-190 (BREITWIGNER and NONRESONANT types)
+void GetAmpLz(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, cd* H) {
+    for (int ir = 0; ir < M.nR; ++ir) {
+        double m0 = ap.m0[ir];
+        const double g0 = ap.g0[ir];
+        const int S2 = M.d->spin2[ir];
+        const int LR = ap.L[ir];
+        const double S = 0.5 * S2;
+        const int type = M.d->type[ir];
+        cd BW(0, 0);
+        if (type == PWA_BREITWIGNER) BW = BW_AMP(P, m0, g0, z.mpk, LR);
+        else if (type == PWA_NONRESONANT) BW = cd(1.0, 0.0);
+        BW = BTerm(P, S, z.mpk, m0) * BW;
+        int ampcount = ap.ampcount[ir];
+        const int spin_index = (S2 - 1) / 2;
+        for (int k = 0; k <= 1; ++k)
+            for (int m = 0; m < 6; ++m) {
+                if (std::fabs(S - 0.5) < 1e-10 && m >= 4) continue;
+                const cd c = ap.vamp[ampcount++];
+                if (std::abs(c) < 1e-10) continue;
+                for (int i = 0; i <= 1; ++i)
+                    for (int j = 0; j <= 1; ++j)
+                        H[4 * i + 2 * j + k] += z.DL[spin_index][k][m] * (z.A[i][j][m] * BW) * c;
+            }
+    }
+}
+
+// GetAmpZcs, src/kernel_calEva.cu:192-286.  The reference loop steps hel_p
+// over {0,1} (:226), which disagrees with the coupling layout of
+// cuda_configure_ampcount (:1066-1087) and the tables of filldlz
+// (:7250-7302); the Zcs chain is never compiled in the shipped configuration
+// (NX = 0, RooTDalitz/NDefine.h:10).  Restated here with hel_p = +-1/2 as the
+// layout implies: couplings and DX packed over allowed (helpsi, hel_p, hel_Z),
+// AX indexed by the full (helpsi, hel_p, hel_Z) position.
+void GetAmpZcs(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, cd* H) {
+    for (int iX = 0; iX < M.nX; ++iX) {
+        const int ir = M.nR + M.nZ + iX;
+        const double m0 = ap.m0[ir], g0 = ap.g0[ir];
+        const int S2 = M.d->spin2[ir];
+        const int LR = ap.L[ir];
+        const double S = 0.5 * S2;
+        cd BW = BTermX(P, S, z.mjpsik, m0) * BW_AMP2(P.m_k, P.m_jpsi, P.ffr, m0, g0, z.mjpsik, LR);
+        int ampcount = ap.ampcount[ir];
+        const int spin_index = S2 / 2;
+        for (int k = 0; k <= 1; ++k) {
+            int m = 0;
+            for (int helpsi = -1; helpsi <= 1; ++helpsi) {
+                if (std::abs(helpsi) > S) continue;
+                for (int ihp = 0; ihp <= 1; ++ihp) {
+                    double help = (ihp == 0) ? -0.5 : 0.5;
+                    for (int ihz = 0; ihz <= 1; ++ihz) {
+                        double helZ = (ihz == 0) ? help - 0.5 : help + 0.5;
+                        if (std::fabs(helZ) > S) continue;
+                        const int mfull = (helpsi + 1) * 4 + ihp * 2 + ihz;
+                        const cd c = ap.vamp[ampcount++];
+                        if (std::abs(c) < 1e-10) { m++; continue; }
+                        for (int i = 0; i <= 1; ++i)
+                            for (int j = 0; j <= 1; ++j)
+                                H[4 * i + 2 * j + k] += z.DX[spin_index][k][m] * (z.AX[i][j][mfull] * BW) * c;
+                        m++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Pc lineshape for one resonance at mass mpp (the type switch of
+// src/kernel_calEva.cu:389-451 / :707-760), including BTermZ.
+cd PcLineshape(const Model& M, const AmpParams& ap, const Phys& P, int iPc, double mpp, cd AKM[4][8], bool clamp) {
+    const int ir = M.nR + iPc;
+    double m0 = ap.m0[ir];
+    const double g0 = ap.g0[ir];
+    const int S2 = M.d->spin2[ir];
+    const int LR = ap.L[ir];
+    const double S = 0.5 * S2;
+    const int type = M.d->type[ir];
+    cd BW(0, 0);
+    if (type == PWA_BREITWIGNER) {
+        BW = BW_AMP2(P.m_p, P.m_jpsi, P.ffr, m0, g0, mpp, LR);
+    } else if (type == PWA_KMATRIX) {
+        const int ik = km_class(S2, LR);
+        if (std::abs(AKM[ik][0]) < 1e-10) KMatrix(M, ap, P, mpp, S2, (LR > 0) ? 1 : -1, ik, AKM[ik], clamp);
+        BW = AKM[ik][ap.ikmcount[ir]];
+    } else if (type == PWA_NONRESONANT) {
+        BW = cd(1.0, 0.0);
+    }
+    return BTermZ(P, S, mpp, m0) * BW;
+}
+
+// Pc angular-times-coupling sums without lineshape, HZ[iPc][h]
+// (src/kernel_calEva.cu:610-662; same loop as GetAmp2s :453-495).
+void PcAngular(const Model& M, const AmpParams& ap, const Dlz& z, int iPc, cd* HZ) {
+    const int ir = M.nR + iPc;
+    const int S2 = M.d->spin2[ir];
+    const double S = 0.5 * S2;
+    int ampcount = ap.ampcount[ir];
+    const int spin_index = (S2 - 1) / 2;
+    for (int h = 0; h < 8; ++h) HZ[h] = 0;
+    for (int k = 0; k <= 1; ++k) {
+        int m = 0;
+        for (int hel_psi = -1; hel_psi <= 1; ++hel_psi)
+            for (int ihz = 0; ihz <= 1; ++ihz)
+                for (int ihp = 0; ihp <= 1; ++ihp) {
+                    double hel_p = ihp - 0.5;
+                    if (std::fabs(hel_psi - hel_p) > S) { m++; continue; }
+                    const cd c = ap.vamp[ampcount++];
+                    if (std::abs(c) < 1e-10) { m++; continue; }
+                    for (int i = 0; i <= 1; ++i)
+                        for (int j = 0; j <= 1; ++j)
+                            HZ[4 * i + 2 * j + k] += z.DZ[spin_index][k][m] * z.AZ[i][j][m] * c;
+                    m++;
+                }
+    }
+}
+
+// GetAmp2s<..., interference>, src/kernel_calEva.cu:288-515 (unsmeared)
+double GetAmp2s(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, bool interference, bool clamp) {
+    cd HL[8], HX[8], HZ[8];
+    for (int h = 0; h < 8; ++h) HL[h] = HX[h] = HZ[h] = 0;
+    GetAmpLz(M, ap, P, z, HL);
+    GetAmpZcs(M, ap, P, z, HX);
+    cd AKM[4][8];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 8; ++b) AKM[a][b] = 0;
+    for (int iPc = 0; iPc < M.nZ; ++iPc) {
+        cd hz[8];
+        PcAngular(M, ap, z, iPc, hz);
+        cd BW = PcLineshape(M, ap, P, iPc, z.mjpsip, AKM, clamp);
+        for (int h = 0; h < 8; ++h) HZ[h] += hz[h] * BW;
+    }
+    double tot = 0;
+    if (!interference) {
+        for (int h = 0; h < 8; ++h) tot += std::norm(HL[h]) + std::norm(HX[h]) + std::norm(HZ[h]);
+    } else {
+        for (int h = 0; h < 8; ++h) tot += std::norm(HL[h] + HX[h] + HZ[h]);
+    }
+    return tot;
+}
+
+// TemplateGetAmp2s<NBINS, scheme>, src/kernel_calEva.cu:517-811 (SMARTBINNING)
+double GetAmp2sSmear(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, int scheme, bool clamp) {
+    const pwa_model_desc* d = M.d;
+    const int NZ = M.nZ;
+    cd HLX[8];
+    for (int h = 0; h < 8; ++h) HLX[h] = 0;
+    GetAmpLz(M, ap, P, z, HLX);
+    GetAmpZcs(M, ap, P, z, HLX);
+    std::vector<cd> HZ(NZ * 8), BW1(NZ, cd(0, 0)), tmp(NZ);
+    std::vector<double> RR(NZ * NZ, 0), II(NZ * NZ, 0), IR(NZ * NZ, 0), RI(NZ * NZ, 0);
+    for (int iPc = 0; iPc < NZ; ++iPc) PcAngular(M, ap, z, iPc, &HZ[iPc * 8]);
+    const double mjpsip = z.mjpsip;
+    const double mres = d->mres[0] + mjpsip * (d->mres[1] + mjpsip * d->mres[2]);
+    const int NB = d->scheme_nbins[scheme - 1];
+    const double weight_smear = 1. / NB;
+    for (int b = 0; b < NB; ++b) {
+        const double msm = d->scheme_z[scheme - 1][b] * mres;
+        cd AKM[4][8];
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 8; ++c) AKM[a][c] = 0;
+        for (int iPc = 0; iPc < NZ; ++iPc) {
+            cd BW = PcLineshape(M, ap, P, iPc, mjpsip + msm, AKM, clamp) * weight_smear;
+            tmp[iPc] = BW;
+            BW1[iPc] += BW;
+        }
+        const double unscale = 1. / weight_smear;
+        for (int i = 0; i < NZ; ++i) {
+            cd iBW = tmp[i];
+            for (int j = 0; j < NZ; ++j) {
+                cd jBW = unscale * tmp[j];
+                RR[i * NZ + j] += iBW.real() * jBW.real();
+                II[i * NZ + j] += iBW.imag() * jBW.imag();
+                IR[i * NZ + j] += iBW.imag() * jBW.real();
+                RI[i * NZ + j] += iBW.real() * jBW.imag();
+            }
+        }
+    }
+    double tot = 0;
+    for (int k = 0; k <= 1; ++k)
+        for (int i = 0; i <= 1; ++i)
+            for (int j = 0; j <= 1; ++j) {
+                const int h = 4 * i + 2 * j + k;
+                tot += std::norm(HLX[h]);
+                for (int a = 0; a < NZ; ++a) {
+                    cd amp1 = BW1[a] * HZ[a * 8 + h];
+                    tot += 2 * (amp1.real() * HLX[h].real() + amp1.imag() * HLX[h].imag());
+                    for (int b = 0; b < NZ; ++b) {
+                        const cd& x = HZ[a * 8 + h];
+                        const cd& y = HZ[b * 8 + h];
+                        tot += (RR[a * NZ + b] + II[a * NZ + b]) * (x.real() * y.real() + x.imag() * y.imag()) +
+                               (IR[a * NZ + b] - RI[a * NZ + b]) * (x.real() * y.imag() - x.imag() * y.real());
+                    }
+                }
+            }
+    return tot;
+}
+
+bool model_from(const pwa_model_desc* d, Model& M) {
+    if (!d) return false;
+    M.d = d;
+    M.nR = d->n_lambda;
+    M.nZ = d->n_pc;
+    M.nX = d->n_zcs;
+    M.n = M.nR + M.nZ + M.nX;
+    return M.n > 0 && M.n <= PWA_MAX_RES;
+}
+
+Phys phys_from(const pwa_model_desc* d) {
+    return Phys{d->m_lb, d->m_k, d->m_p, d->m_jpsi, d->ffr, d->ffb};
+}
+
+double event_amp2(const Model& M, const AmpParams& ap, const Phys& P, const double* ev, int scheme, bool interference) {
+    Dlz z;
+    filldlz(z, ev);
+    const bool clamp = (M.d->flags & PWA_F_KM_CLAMP) != 0;
+    if (scheme > 0 && interference) return GetAmp2sSmear(M, ap, P, z, scheme, clamp);
+    return GetAmp2s(M, ap, P, z, interference, clamp);
+}
+
+void gather(const double* cols, int64_t ld, int64_t i, double* ev) {
+    for (int c = 0; c < PWA_NCOL; ++c) ev[c] = cols[c * ld + i];
+}
+
+} // namespace
+
+// ======================================================================
+//                               C API
+// ======================================================================
+extern "C" {
+
+double orc_dfun(double beta, double j, double m, double n) { return dfun(beta, j, m, n); }
+double orc_dfunhalf(double beta, int m2, int n2) { return dfunhalf(beta, m2, n2); }
+double orc_dfunone(double beta, int m, int n) { return dfunone(beta, m, n); }
+double orc_cg(int J, int M, int j1, int j2, int m1, int m2) { return cg(J, M, j1, j2, m1, m2); }
+double orc_c2(int iS, int iL) { return C2(iS, iL); }
+double orc_getp(double M, double m1, double m2) { return getp(M, m1, m2); }
+double orc_fbw(double r, int L, double p, double p0) { return F_BW(r, L, p, p0); }
+double orc_getfbw2(double M2, double m1, double m2, int L, double ffb) { return getFBW2(M2, m1, m2, L, ffb); }
+void orc_csqrt(double re, double im, double* out) {
+    cd v = csqrt_ref(cd(re, im));
+    out[0] = v.real();
+    out[1] = v.imag();
+}
+void orc_breit_wigner(double m, double g, double s, double* out) {
+    cd v = breit_wigner(m, g, s);
+    out[0] = v.real();
+    out[1] = v.imag();
+}
+
+// chain: 0 Lambda* (BW_AMP), 1 Pc (BW_AMPZ), 2 Zcs (BW_AMPX)
+void orc_bw_amp(const pwa_model_desc* d, int chain, double m_r, double g_r, double mpp, int LR, double* out) {
+    Phys P = phys_from(d);
+    cd v;
+    if (chain == 0) v = BW_AMP(P, m_r, g_r, mpp, LR);
+    else if (chain == 1) v = BW_AMP2(P.m_p, P.m_jpsi, P.ffr, m_r, g_r, mpp, LR);
+    else v = BW_AMP2(P.m_k, P.m_jpsi, P.ffr, m_r, g_r, mpp, LR);
+    out[0] = v.real();
+    out[1] = v.imag();
+}
+
+double orc_bterm(const pwa_model_desc* d, int chain, double J, double mpp, double mr) {
+    Phys P = phys_from(d);
+    if (chain == 0) return BTerm(P, J, mpp, mr);
+    if (chain == 1) return BTermZ(P, J, mpp, mr);
+    return BTermX(P, J, mpp, mr);
+}
+
+// Row 0 of the inverse of a complex n x n matrix (interleaved re,im, row-major).
+void orc_solve_row0(const double* S, int n, double* x) {
+    cd A[16], X[4];
+    for (int i = 0; i < n * n; ++i) A[i] = cd(S[2 * i], S[2 * i + 1]);
+    solve_row0(A, n, X);
+    for (int i = 0; i < n; ++i) {
+        x[2 * i] = X[i].real();
+        x[2 * i + 1] = X[i].imag();
+    }
+}
+
+int orc_fit_param_count(const pwa_model_desc* d) {
+    Model M;
+    if (!model_from(d, M)) return -1;
+    return M.nR * PWA_NPAR_LAMBDA + M.nZ * PWA_NPAR_PC + M.nX * PWA_NPAR_ZCS;
+}
+
+// Amplitude-level parameters (interleaved complex vamp; km block 4 x 32).
+// Returns the number of complex helicity couplings, or -1.
+int orc_build_amp(const pwa_model_desc* d, const double* par, double* m0, double* g0, int* L, double* vamp,
+                  int vamp_cap, double* km, int* ampcount) {
+    Model M;
+    if (!model_from(d, M)) return -1;
+    AmpParams ap;
+    build_amp(M, par, ap);
+    for (int r = 0; r < M.n; ++r) {
+        if (m0) m0[r] = ap.m0[r];
+        if (g0) g0[r] = ap.g0[r];
+        if (L) L[r] = ap.L[r];
+        if (ampcount) ampcount[r] = ap.ampcount[r];
+    }
+    const int nv = (int)ap.vamp.size();
+    if (vamp) {
+        if (nv > vamp_cap) return -1;
+        for (int i = 0; i < nv; ++i) {
+            vamp[2 * i] = ap.vamp[i].real();
+            vamp[2 * i + 1] = ap.vamp[i].imag();
+        }
+    }
+    if (km)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 32; ++b) km[a * 32 + b] = ap.km[a][b];
+    return nv;
+}
+
+// K-matrix production amplitudes Mtr[0..Npole+3] of class ik at mass mpp.
+int orc_kmatrix(const pwa_model_desc* d, const double* par, double mpp, int ik, double* out) {
+    Model M;
+    if (!model_from(d, M)) return -1;
+    AmpParams ap;
+    build_amp(M, par, ap);
+    Phys P = phys_from(d);
+    cd Mtr[8];
+    const int J = (ik == 1 || ik == 3) ? 3 : 1;
+    const int Pr = (ik >= 2) ? 1 : -1;
+    KMatrix(M, ap, P, mpp, J, Pr, ik, Mtr, (d->flags & PWA_F_KM_CLAMP) != 0);
+    for (int i = 0; i < 8; ++i) {
+        out[2 * i] = Mtr[i].real();
+        out[2 * i + 1] = Mtr[i].imag();
+    }
+    return 0;
+}
+
+// Per-event angular tables, flattened in struct order (for tests).
+void orc_filldlz(const double* ev, double* out) {
+    Dlz z;
+    filldlz(z, ev);
+    int o = 0;
+    out[o++] = z.mpk;
+    out[o++] = z.mjpsip;
+    out[o++] = z.mjpsik;
+    for (auto& a : z.A) for (auto& b : a) for (auto& c : b) { out[o++] = c.real(); out[o++] = c.imag(); }
+    for (auto& a : z.DL) for (auto& b : a) for (auto& c : b) out[o++] = c;
+    for (auto& a : z.AZ) for (auto& b : a) for (auto& c : b) { out[o++] = c.real(); out[o++] = c.imag(); }
+    for (auto& a : z.DZ) for (auto& b : a) for (auto& c : b) out[o++] = c;
+    for (auto& a : z.AX) for (auto& b : a) for (auto& c : b) { out[o++] = c.real(); out[o++] = c.imag(); }
+    for (auto& a : z.DX) for (auto& b : a) for (auto& c : b) out[o++] = c;
+}
+
+// |A|^2 per event. interference = 0 gives the no-interference sum of
+// GetAmp2s<...,false>; scheme > 0 selects the smeared form.
+int orc_amp2(const pwa_model_desc* d, const double* par, int64_t n, const double* cols, int64_t ld,
+             const int32_t* scheme, int interference, double* out) {
+    Model M;
+    if (!model_from(d, M)) return -1;
+    AmpParams ap;
+    build_amp(M, par, ap);
+    Phys P = phys_from(d);
+    double ev[PWA_NCOL];
+    for (int64_t i = 0; i < n; ++i) {
+        gather(cols, ld, i, ev);
+        out[i] = event_amp2(M, ap, P, ev, scheme ? scheme[i] : 0, interference != 0);
+    }
+    return 0;
+}
+
+// NLL as assembled by host_store_fx, src/kernel_calEva.cu:1186-1363, for one
+// device: sumint = sum_mc fx_mc with fx_mc = w*|A|^2 (w <= -1000 -> |A|^2),
+// mynll = NLL_SHIFT + sumW_data*ln(sumint/sumW_mc) - sum_data (fx>0 ? sw*ln fx : -1e12),
+// +1000 when |I_noint/I - 1| > 0.3.  Sums in plain FP64 order.
+int orc_nll(const pwa_model_desc* d, const double* par, int64_t nd, const double* dcols, int64_t dld,
+            const double* sw, const int32_t* dscheme, int64_t nm, const double* mcols, int64_t mld,
+            const double* wmc, const int32_t* mscheme, double* nll, double* integral, double* integral_noint,
+            double* fx_data, double* fx_mc) {
+    Model M;
+    if (!model_from(d, M)) return -1;
+    AmpParams ap;
+    build_amp(M, par, ap);
+    Phys P = phys_from(d);
+    double ev[PWA_NCOL];
+    double sumint = 0, sumnoint = 0, sumW = 0, sumW_data = 0, nll_data = 0;
+    for (int64_t i = 0; i < nm; ++i) {
+        gather(mcols, mld, i, ev);
+        const double a2 = event_amp2(M, ap, P, ev, mscheme ? mscheme[i] : 0, true);
+        const double w = wmc[i];
+        const double fx = (w <= -1000.) ? a2 : a2 * w;
+        sumint += fx;
+        sumW += w;
+        double noint = 0;
+        if (!(w <= -1000.)) noint = event_amp2(M, ap, P, ev, 0, false) * w;
+        sumnoint += noint;
+        if (fx_mc) fx_mc[i] = fx;
+    }
+    const bool negpen = (d->flags & PWA_F_NEG_FX_PENALTY) != 0;
+    for (int64_t i = 0; i < nd; ++i) {
+        gather(dcols, dld, i, ev);
+        const double fx = event_amp2(M, ap, P, ev, dscheme ? dscheme[i] : 0, true);
+        sumW_data += sw[i];
+        double t;
+        if (fx > 0) t = std::log(fx) * sw[i];
+        else t = negpen ? -1e12 : 0.0;
+        nll_data += t;
+        if (fx_data) fx_data[i] = fx;
+    }
+    double v = d->nll_shift + sumW_data * std::log(sumint / sumW) - nll_data;
+    if ((d->flags & PWA_F_NOINT_PENALTY) && std::fabs(sumnoint / sumint - 1.0) > 0.3) v += 1000.0;
+    if (nll) *nll = v;
+    if (integral) *integral = sumint;
+    if (integral_noint) *integral_noint = sumnoint;
+    return 0;
+}
+
+// Event classification of the reference for sorted data
+// (src/RooDalitzAmplitude.cxx:391-397 counts, :1945 passes _Nda_SMR1 - 1;
+// src/kernel_calEva.cu:920-961 ranges; :874 MC/normalisation windows).
+// scheme ids: 1 = 30-bin, 2 = 150-bin, 3 = 20-bin, 0 = unsmeared.
+// The data must already be ordered [30-bin window | fine window | rest]
+// (gen_toys/rearrange_mJpsip/reOrder.C:6-10); n_smr1/n_fine are the counts.
+void orc_reference_schemes(int64_t nd, const double* mjpsip_data, int64_t n_smr1, int64_t n_fine, int64_t nm,
+                           const double* mjpsip_mc, int32_t* dscheme, int32_t* mscheme) {
+    const int64_t nda_smr = n_smr1 - 1;
+    auto norm_scheme = [](double m) {
+        const float mf = (float)m;
+        return ((mf > 4.4 && mf < 4.48) || (mf > 4.27 && mf < 4.34)) ? 3 : 0;
+    };
+    for (int64_t i = 0; i < nd; ++i) {
+        if (i < nda_smr) dscheme[i] = 1;
+        else if (i < nda_smr + n_fine) dscheme[i] = 2;
+        else dscheme[i] = norm_scheme(mjpsip_data[i]);
+    }
+    for (int64_t i = 0; i < nm; ++i) mscheme[i] = norm_scheme(mjpsip_mc[i]);
+}
+
+// FCN, src/fit.cpp:312-317 + RooFGaussian::evaluate src/RooFGaussian.cxx:60-75
+double orc_fcn(double nll, double nll_shift, double alpha, const double* par, int nc, const int* idx,
+               const double* mu1, const double* mu2, const double* s1, const double* s2) {
+    double c = 0;
+    for (int i = 0; i < nc; ++i) {
+        const double x = par[idx[i]];
+        if (x < mu1[i]) {
+            const double a = (x - mu1[i]) / s1[i];
+            c += 0.5 * a * a;
+        } else if (x > mu2[i]) {
+            const double a = (x - mu2[i]) / s2[i];
+            c += 0.5 * a * a;
+        }
+    }
+    return (nll - nll_shift) * alpha + c;
+}
+
+// alpha = sum w / sum w^2, src/fit.cpp:201-220
+double orc_alpha(int64_t n, const double* w) {
+    double s = 0, s2 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        s += w[i];
+        s2 += w[i] * w[i];
+    }
+    return s / s2;
+}
+
+} // extern "C"

@@ -1,0 +1,483 @@
+// Device building blocks of the amplitude (shared by the value and the
+// gradient kernels): per-event angular tables rebuilt from the compact
+// planes, the Lambda*, Pc and Zcs chains, the K-matrix, and the compensated
+// block reduction.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pwa_device.hpp"
+
+namespace pwa {
+
+// ------------------------------------------------------- small helpers
+__device__ __forceinline__ C64 ld_c(const double* ev, int64_t ld, int64_t i, int pr, int pi) {
+    return C64(ev[(int64_t)pr * ld + i], ev[(int64_t)pi * ld + i]);
+}
+__device__ __forceinline__ double ld_d(const double* ev, int64_t ld, int64_t i, int p) { return ev[(int64_t)p * ld + i]; }
+
+// dfunhalf(beta, 2a, 2b) from c = cos(beta/2), s = sin(beta/2)
+__device__ __forceinline__ double dhalf(int a2, int b2, double c, double s) {
+    return (a2 == b2) ? c : (a2 > b2 ? -s : s);
+}
+// dfunone(beta, m, n) for n = +-1 from cos(beta), sin(beta)
+__device__ __forceinline__ double done(int m, int n, double cb, double sb) {
+    const double r2 = 0.70710678118654752440;
+    if (m == 0) return (n > 0 ? 1.0 : -1.0) * sb * r2;
+    return (m == n) ? 0.5 * (1.0 + cb) : 0.5 * (1.0 - cb);
+}
+
+// Lambda* Wigner table d^J_{lambda1(m), lambda_p(k)}(theta_L), m = 0..5 with
+// 2*lambda1 = {1,1,-1,-1,3,-3}, k = 0,1 with 2*lambda_p = {-1,+1}.
+template <int J2>
+__device__ __forceinline__ void dl_table(const double* cp, const double* sp, double dl[2][6]) {
+    const double a0 = wigner_d<J2, 1, -1>(cp, sp), a1 = wigner_d<J2, 1, 1>(cp, sp);
+    const double b0 = wigner_d<J2, -1, -1>(cp, sp), b1 = wigner_d<J2, -1, 1>(cp, sp);
+    const double c0 = wigner_d<J2, 3, -1>(cp, sp), c1 = wigner_d<J2, 3, 1>(cp, sp);
+    const double e0 = wigner_d<J2, -3, -1>(cp, sp), e1 = wigner_d<J2, -3, 1>(cp, sp);
+    dl[0][0] = a0; dl[0][1] = a0; dl[0][2] = b0; dl[0][3] = b0; dl[0][4] = c0; dl[0][5] = e0;
+    dl[1][0] = a1; dl[1][1] = a1; dl[1][2] = b1; dl[1][3] = b1; dl[1][4] = c1; dl[1][5] = e1;
+}
+
+// Pc Wigner values d^J_{helZ, n}(theta_Z): dz[iz][in], 2*helZ = {-1,1}, 2*n = {-3,-1,1,3}.
+template <int J2>
+__device__ __forceinline__ void dz_table(const double* cp, const double* sp, double dz[2][4]) {
+    dz[0][0] = wigner_d<J2, -1, -3>(cp, sp); dz[0][1] = wigner_d<J2, -1, -1>(cp, sp);
+    dz[0][2] = wigner_d<J2, -1, 1>(cp, sp);  dz[0][3] = wigner_d<J2, -1, 3>(cp, sp);
+    dz[1][0] = wigner_d<J2, 1, -3>(cp, sp);  dz[1][1] = wigner_d<J2, 1, -1>(cp, sp);
+    dz[1][2] = wigner_d<J2, 1, 1>(cp, sp);   dz[1][3] = wigner_d<J2, 1, 3>(cp, sp);
+}
+
+// Zcs Wigner values d^J_{helZ, -helpsi}(theta_X): dx[helZ+1][-helpsi+1], integer J.
+template <int J2>
+__device__ __forceinline__ void dx_table(const double* cp, const double* sp, double dx[3][3]) {
+#define PWA_DX(a, b) dx[(a) / 2 + 1][(b) / 2 + 1] = wigner_d<J2, a, b>(cp, sp);
+    PWA_DX(-2, -2) PWA_DX(-2, 0) PWA_DX(-2, 2)
+    PWA_DX(0, -2) PWA_DX(0, 0) PWA_DX(0, 2)
+    PWA_DX(2, -2) PWA_DX(2, 0) PWA_DX(2, 2)
+#undef PWA_DX
+}
+
+__device__ __forceinline__ C64 cpow_int(C64 e, int n) {
+    // e^n for n in {-1, 0, 1} with |e| = 1
+    if (n == 0) return C64(1.0, 0.0);
+    return n > 0 ? e : conj(e);
+}
+
+// ------------------------------------------------------ Lambda* chain
+// H[4i+2j+k] += sum_r L_r sum_m c_{r,k,m} DL_J[k][m] A[i][j][m]   (GetAmpLz, src/kernel_calEva.cu:85-190)
+__device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                             const C64* __restrict__ cpl, const double* __restrict__ ev,
+                                             int64_t ld, int64_t i, double mkp, C64 H[8]) {
+    const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
+    const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
+    const double cL = ld_d(ev, ld, i, E_L_CL), sL = ld_d(ev, ld, i, E_L_SL);
+    const C64 eK = ld_c(ev, ld, i, E_L_EKR, E_L_EKI);
+    const C64 eM = ld_c(ev, ld, i, E_L_EMR, E_L_EMI);
+    double cp[10], sp[10];
+    powers(cL, cp, 9);
+    powers(sL, sp, 9);
+
+    C64 G[2][6];
+    for (int g = 0; g < 5; ++g) {
+        const int beg = S->lam_start[g], end = S->lam_start[g + 1];
+        if (beg == end) continue;
+        double dl[2][6];
+        switch (g) {
+        case 0: dl_table<1>(cp, sp, dl); break;
+        case 1: dl_table<3>(cp, sp, dl); break;
+        case 2: dl_table<5>(cp, sp, dl); break;
+        case 3: dl_table<7>(cp, sp, dl); break;
+        default: dl_table<9>(cp, sp, dl); break;
+        }
+        const int nm = (g == 0) ? 4 : 6;
+        for (int q = beg; q < end; ++q) {
+            const int r = S->lam_idx[q];
+            const double m0 = D->m0[r], g0 = D->g0[r];
+            C64 Ls(1.0, 0.0);
+            if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<double>(S->P, m0, g0, mkp, S->L[r]);
+            Ls = Ls * vertex_lambda<double>(S->P, S->spin2[r], mkp, m0);
+            const C64* c = cpl + S->ampcount[r];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    if (m < nm) rmac(G[k][m], dl[k][m], Ls * c[k * nm + m]);
+                }
+        }
+    }
+    // A[i][j][m] = e^{i(l1 phiK + l2 phiMu)} dfunhalf(theta_Lb, 2 l0, 2(l1-l2)) dfunone(theta_Jpsi, l2, dmu)
+    const C64 eK3 = eK * eK * eK;
+    const C64 ph[6] = {eK * eM, eK, conj(eK), conj(eK) * conj(eM), eK3 * eM, conj(eK3) * conj(eM)};
+    const int l2[6] = {1, 0, 0, -1, 1, -1};
+    const int dl2[6] = {-1, 1, -1, 1, 1, -1};  // 2(l1 - l2)
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        C64 P[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) P[m] = ph[m] * G[k][m];
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                C64 acc;
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+                    rmac(acc, dhalf(2 * jj - 1, dl2[m], cB, sB) * done(l2[m], 2 * ii - 1, cJ, sJ), P[m]);
+                H[4 * ii + 2 * jj + k] = H[4 * ii + 2 * jj + k] + acc;
+            }
+    }
+}
+
+// --------------------------------------------- two-body chain angular part
+// Shared structure of the Pc and Zcs chains: S[4i+2j+k] = e^{i(l0 phiR + dmu alpha)}
+//   sum_{psi, d} dfunone(theta_psi; psi, dmu) dfunhalf(theta_B; 2 l0, 2d)
+//                e^{i(psi phiMu + d phiPsi)} V[k][psi][d]
+// where V collects couplings times the chain's Wigner factors.
+struct ChainAng {
+    double cB, sB, cJ, sJ;
+    C64 eR, eM, eP, eA;
+};
+__device__ __forceinline__ void chain_contract(const ChainAng& a, const C64 V[2][3][2], C64 S[8]) {
+    C64 W[2][3][2];
+#pragma unroll
+    for (int ps = 0; ps < 3; ++ps)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const C64 q = cpow_int(a.eM, ps - 1) * (d ? a.eP : conj(a.eP));
+#pragma unroll
+            for (int k = 0; k < 2; ++k) W[k][ps][d] = q * V[k][ps][d];
+        }
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+            const C64 pij = (jj ? a.eR : conj(a.eR)) * (ii ? a.eA : conj(a.eA));
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                C64 acc;
+#pragma unroll
+                for (int ps = 0; ps < 3; ++ps)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d)
+                        rmac(acc, done(ps - 1, 2 * ii - 1, a.cJ, a.sJ) * dhalf(2 * jj - 1, 2 * d - 1, a.cB, a.sB), W[k][ps][d]);
+                S[4 * ii + 2 * jj + k] = pij * acc;
+            }
+        }
+}
+
+__device__ __forceinline__ ChainAng load_chain(const double* ev, int64_t ld, int64_t i, int base) {
+    ChainAng a;
+    a.cB = ld_d(ev, ld, i, base + 0);
+    a.sB = ld_d(ev, ld, i, base + 1);
+    a.cJ = ld_d(ev, ld, i, base + 2);
+    a.sJ = ld_d(ev, ld, i, base + 3);
+    a.eR = ld_c(ev, ld, i, base + 8, base + 9);
+    a.eM = ld_c(ev, ld, i, base + 10, base + 11);
+    a.eP = ld_c(ev, ld, i, base + 12, base + 13);
+    a.eA = ld_c(ev, ld, i, base + 14, base + 15);
+    return a;
+}
+
+// Pc angular-times-coupling sums (no lineshape) for resonance r
+// (src/kernel_calEva.cu:610-662; tables of filldlz src/RooDalitzAmplitude.cxx:7189-7245).
+__device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
+                                           const ChainAng& a, const double* cp, const double* sp, double cP,
+                                           double sP, C64 out[8]) {
+    const int J2 = S->spin2[r];
+    double dz[2][4];
+    switch (J2) {
+    case 1: dz_table<1>(cp, sp, dz); break;
+    case 3: dz_table<3>(cp, sp, dz); break;
+    case 5: dz_table<5>(cp, sp, dz); break;
+    default: dz_table<7>(cp, sp, dz); break;
+    }
+    C64 V[2][3][2];
+    const C64* c = cpl + S->ampcount[r];
+    int n = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int ps = 0; ps < 3; ++ps)
+            for (int iz = 0; iz < 2; ++iz)
+                for (int ip = 0; ip < 2; ++ip) {
+                    const int n2 = 2 * (ps - 1) - (2 * ip - 1);  // 2(helpsi - help)
+                    if (n2 > J2 || n2 < -J2) continue;
+                    const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
+                    rmac(V[k][ps][iz], w, c[n]);
+                    ++n;
+                }
+    chain_contract(a, V, out);
+}
+
+// Zcs chain: H += L_X * S_X   (GetAmpZcs, src/kernel_calEva.cu:192-286, with
+// the coupling layout of cuda_configure_ampcount :1066-1087)
+__device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                          const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld,
+                                          int64_t i, double mjk, C64 H[8]) {
+    const ChainAng a = load_chain(ev, ld, i, E_X_CB);
+    const double cX = ld_d(ev, ld, i, E_X_CZ), sX = ld_d(ev, ld, i, E_X_SZ);
+    const double cP = ld_d(ev, ld, i, E_X_CP), sP = ld_d(ev, ld, i, E_X_SP);
+    double cp[8], sp[8];
+    powers(cX, cp, 7);
+    powers(sX, sp, 7);
+    for (int q = 0; q < S->nX; ++q) {
+        const int r = S->nR + S->nZ + q;
+        const int J2 = S->spin2[r];
+        double dx[3][3];
+        switch (J2) {
+        case 0: dx_table<0>(cp, sp, dx); break;
+        case 2: dx_table<2>(cp, sp, dx); break;
+        case 4: dx_table<4>(cp, sp, dx); break;
+        default: dx_table<6>(cp, sp, dx); break;
+        }
+        C64 V[2][3][2];
+        const C64* c = cpl + S->ampcount[r];
+        int n = 0;
+        for (int k = 0; k < 2; ++k)
+            for (int ps = 0; ps < 3; ++ps) {
+                if (2 * abs(ps - 1) > J2) continue;
+                for (int ip = 0; ip < 2; ++ip)
+                    for (int iz = 0; iz < 2; ++iz) {
+                        const int hz2 = (2 * ip - 1) + (2 * iz - 1);  // 2 helZ = 2 help -+ 1
+                        if (hz2 > J2 || hz2 < -J2) continue;
+                        const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dx[hz2 / 2 + 1][(1 - ps) + 1];
+                        rmac(V[k][ps][iz], w, c[n]);
+                        ++n;
+                    }
+            }
+        C64 Sx[8];
+        chain_contract(a, V, Sx);
+        const double m0 = D->m0[r], g0 = D->g0[r];
+        C64 Ls = bw_twobody<double>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
+        Ls = Ls * vertex_zcs<double>(S->P, J2, mjk, m0);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) cmac(H[h], Ls, Sx[h]);
+    }
+}
+
+// ----------------------------------------------------------- K-matrix
+__device__ __forceinline__ C64 rho_real(double s, double m1, double m2, int L, double ffb) {
+    const double rho2 = (s - (m1 + m2) * (m1 + m2)) * (s - (m1 - m2) * (m1 - m2));
+    const double rho = sqrt(fabs(rho2)) / s * fbw2(s, m1, m2, L, ffb);
+    return rho2 >= 0 ? C64(rho, 0.0) : C64(0.0, rho);
+}
+__device__ __forceinline__ C64 rho_cmass(double s, double m1, double m2, double hw) {
+    const C64 a(m1 + m2, -hw), b(m1 - m2, hw);
+    const C64 p1 = a * a - C64(s, 0.0), p2 = b * b - C64(s, 0.0);
+    const C64 r = csqrt_branch(p1 * p2);
+    return C64(r.re / s, r.im / s);
+}
+
+// Production amplitudes of K-matrix class ik at mass mpp: Mtr[p] (poles) and
+// Mtr[np + c] (non-resonant channels), as Default_KMatrix
+// (src/cu_DPFPropogator.cu:588-813): T = row 0 of (D (1 - i K rho))^-1 with
+// K = sum_p g_p g_p^T / (m_p^2 - s) + f and D = prod_p (m_p^2 - s).
+static __device__ void kmatrix(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int ik, double mpp,
+                        C64 Mtr[8]) {
+    const int Lna = (ik >= 2) ? 1 : 0;
+    const double s = mpp * mpp;
+    C64 rho[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        rho[c] = S->km_cplx[ik][c] ? rho_cmass(s, S->km_ma[ik][c], S->km_mb[ik][c], S->km_hw[ik][c])
+                                   : rho_real(s, S->km_ma[ik][c], S->km_mb[ik][c], Lna, S->P.ffb);
+    const double* g = D->km[ik];
+    const double* f = D->km[ik] + 16;
+    const int np = D->km_npole[ik];
+    double den[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        const int r = D->km_pole_res[ik][p];
+        const double mfit = D->m0[r];
+        const int ch = D->km_pole_shift[ik][p];
+        double m02 = mfit * mfit;
+        if (ch > 0) {
+            const C64 rc = rho_cmass(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch]);
+            const double gc = g[p * 4 + ch];
+            const double m0 = sqrt(m02 - gc * gc * rc.im);
+            m02 = m0 * m0;
+        }
+        double d = m02 - s;
+        if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) d = 1e-3;
+        den[p] = d;
+    }
+    double Dall = 1.0, ms[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) Dall *= (p < np) ? den[p] : 1.0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double t = 1.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q != p && q < np) t *= den[q];
+        ms[p] = t;
+    }
+    double kma[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b) {
+            double v = f[a * 4 + b] * Dall;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (p < np) v += g[p * 4 + a] * (g[p * 4 + b] * ms[p]);
+            kma[a][b] = v;
+            kma[b][a] = v;
+        }
+    // A = KK^T, KK[a][b] = D delta_ab - i kma[a][b] rho[b]
+    C64 A[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            // A[a][b] = KK[b][a] = -i kma[b][a] rho[a] + D delta
+            const C64 t = rho[a] * kma[b][a];
+            A[a][b] = C64(t.im, -t.re);
+            if (a == b) A[a][b].re += Dall;
+        }
+    // Doolittle LU without pivoting, then solve A x = e0
+    C64 Lm[4][4], U[4][4], iU[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int col = a; col < 4; ++col) {
+            C64 u = A[a][col];
+#pragma unroll
+            for (int k = 0; k < a; ++k) u = u - Lm[a][k] * U[k][col];
+            U[a][col] = u;
+        }
+        iU[a] = cinv(U[a][a]);
+#pragma unroll
+        for (int row = a + 1; row < 4; ++row) {
+            C64 l = A[row][a];
+#pragma unroll
+            for (int k = 0; k < a; ++k) l = l - Lm[row][k] * U[k][a];
+            Lm[row][a] = l * iU[a];
+        }
+    }
+    C64 y[4], T[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        C64 v = (a == 0) ? C64(1.0, 0.0) : C64(0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < a; ++b) v = v - Lm[a][b] * y[b];
+        y[a] = v;
+    }
+#pragma unroll
+    for (int a = 3; a >= 0; --a) {
+        C64 v = y[a];
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b) v = v - U[a][b] * T[b];
+        T[a] = iU[a] * v;
+    }
+    const double na = Lna ? fbw2(s, S->km_ma[ik][0], S->km_mb[ik][0], Lna, S->P.ffb) : 1.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) Mtr[p] = C64(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        C64 acc;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * ms[p], T[c]);
+        Mtr[p] = acc * na;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const C64 v = T[c] * (Dall * na);
+        // Mtr[np + c] = v, written with compile-time indices only
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == np + c) Mtr[q] = v;
+    }
+}
+
+__device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
+    C64 r = v[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q)
+        if (q == idx) r = v[q];
+    return r;
+}
+
+// Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.
+template <int NZ>
+__device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                              double m, C64 L[NZ]) {
+    const int nR = S->nR;
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) L[r] = C64(1.0, 0.0);
+    for (int c = 0; c < S->km_nclass; ++c) {
+        const int ik = S->km_class[c];
+        C64 Mtr[8];
+        kmatrix(S, D, ik, m, Mtr);
+#pragma unroll
+        for (int r = 0; r < NZ; ++r)
+            if (S->ik[nR + r] == ik) L[r] = select8(Mtr, S->ikm[nR + r]);
+    }
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+        const int rr = nR + r;
+        const double m0 = D->m0[rr];
+        if (S->type[rr] == PWA_BREITWIGNER)
+            L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);
+        L[r] = L[r] * vertex_pc<double>(S->P, S->spin2[rr], m, m0);
+    }
+}
+
+// Quadratic form 2 Re(sum_r L_r p0_r) + sum_{r r'} L_r conj(L_r') M[r'][r]
+template <int NZ>
+__device__ __forceinline__ double pc_form(const C64 L[NZ], const C64 p0[NZ], const double Md[NZ],
+                                          const C64 Mo[NZ][NZ]) {
+    double f = 0.0;
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+        f += 2.0 * (L[r].re * p0[r].re - L[r].im * p0[r].im);
+        f += norm2(L[r]) * Md[r];
+#pragma unroll
+        for (int q = 0; q < r; ++q) {
+            // 2 Re(L_r conj(L_q) M[q][r]) with M[q][r] = sum_h conj(S_q) S_r
+            const C64 lq = L[r] * conj(L[q]);
+            f += 2.0 * (lq.re * Mo[q][r].re - lq.im * Mo[q][r].im);
+        }
+    }
+    return f;
+}
+
+// --------------------------------------------------------- reduction
+__device__ __forceinline__ void kadd(double& s, double& c, double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ void block_reduce2(double& s0, double& c0, double& s1, double& c1) {
+    __shared__ double red[4][kBlock / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double a0 = __shfl_xor(s0, off), b0 = __shfl_xor(c0, off);
+        const double a1 = __shfl_xor(s1, off), b1 = __shfl_xor(c1, off);
+        kadd(s0, c0, a0);
+        c0 += b0;
+        kadd(s1, c1, a1);
+        c1 += b1;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = s0;
+        red[1][w] = c0;
+        red[2][w] = s1;
+        red[3][w] = c1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S0 = 0, C0 = 0, S1 = 0, C1 = 0;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            kadd(S0, C0, red[0][q]);
+            C0 += red[1][q];
+            kadd(S1, C1, red[2][q]);
+            C1 += red[3][q];
+        }
+        s0 = S0; c0 = C0; s1 = S1; c1 = C1;
+    }
+}
+
+} // namespace pwa

@@ -1,0 +1,102 @@
+// Device-side data layout shared by the host orchestration (pwa_host.cpp)
+// and the kernels (pwa_kernels.hip).
+#pragma once
+#include <stdint.h>
+
+#include "pwa_amp.h"
+#include "pwa_math.hpp"
+
+namespace pwa {
+
+// Per-event compact planes (FP64 SoA, plane p of event i at ev[p*ld + i]).
+// Angles enter only through the cosines and sines the amplitude uses:
+// half-angle cos/sin for spin-1/2 and Wigner-d rotations, full-angle cos/sin
+// for spin-1 rotations, and unit phases e^{i phi/2}, e^{i phi} for azimuths.
+enum Plane {
+    E_MKP = 0, E_MJPSIP, E_MJPSIK, E_W,
+    // Lambda* chain
+    E_L_CB, E_L_SB,     // cos, sin (theta_Lb / 2)
+    E_L_CJ, E_L_SJ,     // cos, sin (theta_Jpsi)
+    E_L_CL, E_L_SL,     // cos, sin (theta_L / 2)
+    E_L_EKR, E_L_EKI,   // e^{i phiK / 2}
+    E_L_EMR, E_L_EMI,   // e^{i phiMu}
+    // Pc chain
+    E_Z_CB, E_Z_SB,     // cos, sin (Z_theta_Lb / 2)
+    E_Z_CJ, E_Z_SJ,     // cos, sin (Z_theta_Jpsi)
+    E_Z_CZ, E_Z_SZ,     // cos, sin (Z_theta_Z / 2)
+    E_Z_CP, E_Z_SP,     // cos, sin (theta_p / 2)
+    E_Z_EZR, E_Z_EZI,   // e^{i Z_phiZ / 2}
+    E_Z_EMR, E_Z_EMI,   // e^{i Z_phiMu}
+    E_Z_EPR, E_Z_EPI,   // e^{i Z_phiJpsi / 2}
+    E_Z_EAR, E_Z_EAI,   // e^{i alpha_Mu}
+    // Zcs chain
+    E_X_CB, E_X_SB,
+    E_X_CJ, E_X_SJ,
+    E_X_CZ, E_X_SZ,
+    E_X_CP, E_X_SP,
+    E_X_EZR, E_X_EZI,
+    E_X_EMR, E_X_EMI,
+    E_X_EPR, E_X_EPI,
+    E_X_EAR, E_X_EAI,
+    E_NPLANE
+};
+
+// Static model tables, uploaded once per context.
+struct DevStatic {
+    Masses P;
+    int32_t nR, nZ, nX;
+    uint32_t flags;
+    int32_t spin2[PWA_MAX_RES];
+    int32_t type[PWA_MAX_RES];
+    int32_t L[PWA_MAX_RES];
+    int32_t ampcount[PWA_MAX_RES];  // first complex coupling of resonance r
+    int32_t ik[PWA_MAX_RES];        // K-matrix J^P class of a Pc member, else -1
+    int32_t ikm[PWA_MAX_RES];       // index into the class's production vector
+    // Lambda* resonances grouped by 2J (1,3,5,7,9): lam_idx[lam_start[g] .. lam_start[g+1])
+    int32_t lam_start[6];
+    int32_t lam_idx[PWA_MAX_RES];
+    // K-matrix classes in use and their channel tables
+    int32_t km_nclass;
+    int32_t km_class[PWA_KM_NCLASS];
+    double km_ma[PWA_KM_NCLASS][PWA_KM_NCH], km_mb[PWA_KM_NCLASS][PWA_KM_NCH], km_hw[PWA_KM_NCLASS][PWA_KM_NCH];
+    int32_t km_cplx[PWA_KM_NCLASS][PWA_KM_NCH];
+    double km_shift_ma[PWA_KM_NCH], km_shift_mb[PWA_KM_NCH], km_shift_hw[PWA_KM_NCH];
+    // smearing
+    double mres[3];
+    int32_t scheme_nbins[PWA_MAX_SCHEMES + 1];               // [0] = 1 (unsmeared)
+    double scheme_z[PWA_MAX_SCHEMES + 1][PWA_MAX_SUBBINS];   // [0][0] = 0
+};
+
+// Per-evaluation parameters.
+struct DevDyn {
+    double m0[PWA_MAX_RES];
+    double g0[PWA_MAX_RES];
+    double km[PWA_KM_NCLASS][2 * PWA_KM_NPOLE * PWA_KM_NCH]; // g[pole][ch] then f[ch][ch]
+    // K-matrix poles of each class (members with m0 > 0, in list order):
+    // resonance index and threshold-shift channel (the width slot)
+    int32_t km_npole[PWA_KM_NCLASS];
+    int32_t km_pole_res[PWA_KM_NCLASS][PWA_KM_NPOLE];
+    int32_t km_pole_shift[PWA_KM_NCLASS][PWA_KM_NPOLE];
+};
+
+// Per-block reduction record: compensated sums (hi, lo).
+struct Partial {
+    double s0, c0;  // data: sum sw*ln f        MC: sum w*f
+    double s1, c1;  // data: count of f <= 0     MC: sum w*f_noint
+};
+
+// Launch helpers (pwa_kernels.hip).
+hipError_t launch_prep(const double* raw, int64_t ld_raw, int64_t n, double* ev, int64_t ld, hipStream_t st);
+hipError_t launch_nll(int nZ, bool mc, int scheme, const double* ev, int64_t ld, int64_t n, const DevStatic* S,
+                      const DevDyn* D, const C64* cpl, double* fx, Partial* part, int noint, hipStream_t st);
+hipError_t launch_reduce(const Partial* part, int nseg, const int32_t* seg_begin, const int32_t* seg_kind,
+                         double* out, hipStream_t st);
+hipError_t launch_grad(int nZ, bool mc, int scheme, const double* ev, int64_t ld, int64_t n, const DevStatic* S,
+                       const DevDyn* D, const C64* cpl, int ncpl, int nres, int NG, int blocks, double* gpart,
+                       Partial* part, int noint, hipStream_t st);
+hipError_t launch_grad_reduce(const double* gpart, const int32_t* beg, int NG, double* out, hipStream_t st);
+size_t grad_lds_bytes(int NG);
+constexpr int kBlock = 256;
+constexpr int kGradNClassParams = 36;  // per K-matrix class: g[16], f[16], pole masses[4]
+
+} // namespace pwa

@@ -1,0 +1,650 @@
+// Host orchestration of the MI355X amplitude likelihood: the C ABI of
+// include/pwa_amp.h.  One context = one device, one stream, the model's
+// static tables, the uploaded event samples and the scratch buffers of the
+// evaluation.  The likelihood assembly follows host_store_fx
+// (src/kernel_calEva.cu:1186-1363); the FCN follows src/fit.cpp:201-220,312-317.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pwa_amp.h"
+#include "pwa_couplings.hpp"
+#include "pwa_device.hpp"
+
+using namespace pwa;
+
+namespace {
+
+struct Sample {
+    bool present = false;
+    int64_t n = 0, ld = 0;
+    double* ev = nullptr;               // E_NPLANE x ld, device order
+    double* fx = nullptr;               // per-event output scratch (device order)
+    std::vector<int64_t> perm;          // device position -> caller index
+    std::vector<int> seg_scheme;
+    std::vector<int64_t> seg_begin, seg_count;
+    double sumw = 0.0, sumw2 = 0.0;     // raw weights (data: sw; MC: w)
+};
+
+} // namespace
+
+struct pwa_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    pwa_model_desc desc{};
+    int nres = 0;
+    CouplingMap cmap;
+    DevStatic hs{};
+    DevStatic* dS = nullptr;
+    DevDyn* dD = nullptr;
+    C64* dC = nullptr;
+    int ncpl = 0;
+    Sample smp[2];
+    Partial* part = nullptr;
+    int64_t part_cap = 0;
+    int32_t* dbeg = nullptr;
+    double* dout = nullptr;
+    double* gpart = nullptr;  // per-block partial gradients
+    int64_t gpart_cap = 0;
+    double* dgout = nullptr;  // reduced gradients [2][NG]
+    double* hgout = nullptr;  // pinned copy
+    int NG = 0;
+    double* hout = nullptr;   // pinned
+    DevDyn* hdyn = nullptr;   // pinned
+    C64* hcpl = nullptr;      // pinned
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float last_ms = 0.f;
+    std::string err;
+    std::vector<int> cidx;
+    std::vector<double> cmu1, cmu2, cs1, cs2;
+};
+
+namespace {
+
+int fail(pwa_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                           \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(ctx, PWA_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+int orbital_L(const pwa_model_desc& d, int r) {
+    const int S2 = d.spin2[r], P = d.parity[r];
+    int LR;
+    if (r < d.n_lambda) {
+        LR = (S2 - 1) / 2;
+        if ((((LR + 1) % 2) ? -1 : 1) * P < 0) LR += 1;
+    } else if (r < d.n_lambda + d.n_pc) {
+        LR = std::max((S2 - 3) / 2, 0);
+        if (S2 < 3) LR = 0;
+        if ((((LR + 1) % 2) ? -1 : 1) * P < 0) LR += 1;
+    } else {
+        LR = std::abs(S2 / 2 - 1);
+        if (((LR % 2) ? -1 : 1) * P < 0) LR += 1;
+    }
+    return LR;
+}
+
+int param_offset(const pwa_model_desc& d, int r) {
+    if (r < d.n_lambda) return r * PWA_NPAR_LAMBDA;
+    if (r < d.n_lambda + d.n_pc) return d.n_lambda * PWA_NPAR_LAMBDA + (r - d.n_lambda) * PWA_NPAR_PC;
+    return d.n_lambda * PWA_NPAR_LAMBDA + d.n_pc * PWA_NPAR_PC + (r - d.n_lambda - d.n_pc) * PWA_NPAR_ZCS;
+}
+int param_size(const pwa_model_desc& d, int r) {
+    if (r < d.n_lambda) return PWA_NPAR_LAMBDA;
+    if (r < d.n_lambda + d.n_pc) return PWA_NPAR_PC;
+    return PWA_NPAR_ZCS;
+}
+
+int km_class_of(int S2, int LR) { return (LR > 0) ? ((S2 == 3) ? 3 : 2) : ((S2 == 3) ? 1 : 0); }
+
+int validate(const pwa_model_desc& d, std::string& err) {
+    if (d.n_lambda < 0 || d.n_pc < 0 || d.n_zcs < 0) { err = "negative resonance count"; return PWA_E_MODEL; }
+    const int n = d.n_lambda + d.n_pc + d.n_zcs;
+    if (n <= 0 || n > PWA_MAX_RES) { err = "resonance count out of range"; return PWA_E_MODEL; }
+    if (d.n_pc > 8) { err = "at most 8 Pc-chain resonances are supported"; return PWA_E_MODEL; }
+    if (d.n_schemes < 0 || d.n_schemes > PWA_MAX_SCHEMES) { err = "n_schemes out of range"; return PWA_E_MODEL; }
+    for (int s = 0; s < d.n_schemes; ++s)
+        if (d.scheme_nbins[s] < 1 || d.scheme_nbins[s] > PWA_MAX_SUBBINS) { err = "scheme_nbins out of range"; return PWA_E_MODEL; }
+    int members[4] = {0, 0, 0, 0};
+    for (int r = 0; r < n; ++r) {
+        const int t = d.type[r];
+        if (d.parity[r] != 1 && d.parity[r] != -1) { err = "parity must be +-1"; return PWA_E_MODEL; }
+        const bool pc = r >= d.n_lambda && r < d.n_lambda + d.n_pc;
+        const bool zcs = r >= d.n_lambda + d.n_pc;
+        if (t == PWA_KMATRIX) {
+            if (!pc) { err = "K-matrix lineshape is supported for Pc-chain resonances only"; return PWA_E_MODEL; }
+            if (d.spin2[r] != 1 && d.spin2[r] != 3) { err = "K-matrix members need J = 1/2 or 3/2"; return PWA_E_MODEL; }
+            members[km_class_of(d.spin2[r], orbital_L(d, r))]++;
+        } else if (t == PWA_NONRESONANT) {
+            if (zcs) { err = "non-resonant Zcs is not supported"; return PWA_E_MODEL; }
+        } else if (t != PWA_BREITWIGNER) {
+            err = "unsupported lineshape type " + std::to_string(t);
+            return PWA_E_MODEL;
+        }
+    }
+    for (int k = 0; k < 4; ++k)
+        if (members[k] > 8) { err = "more than 8 members in a K-matrix class"; return PWA_E_MODEL; }
+    return PWA_OK;
+}
+
+void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) {
+    std::memset(&S, 0, sizeof(S));
+    S.P = Masses{d.m_lb, d.m_p, d.m_k, d.m_jpsi, d.ffr, d.ffb};
+    S.nR = d.n_lambda;
+    S.nZ = d.n_pc;
+    S.nX = d.n_zcs;
+    S.flags = d.flags;
+    const int n = S.nR + S.nZ + S.nX;
+    int cnt[4] = {0, 0, 0, 0};
+    bool used[4] = {false, false, false, false};
+    for (int r = 0; r < n; ++r) {
+        S.spin2[r] = d.spin2[r];
+        S.type[r] = d.type[r];
+        S.L[r] = orbital_L(d, r);
+        S.ampcount[r] = cm.ampcount[r];
+        S.ik[r] = -1;
+        S.ikm[r] = -1;
+        if (d.type[r] == PWA_KMATRIX) {
+            const int ik = km_class_of(d.spin2[r], S.L[r]);
+            S.ik[r] = ik;
+            S.ikm[r] = cnt[ik]++;
+            used[ik] = true;
+        }
+    }
+    int q = 0;
+    for (int g = 0; g < 5; ++g) {
+        S.lam_start[g] = q;
+        for (int r = 0; r < S.nR; ++r)
+            if (d.spin2[r] == 2 * g + 1) S.lam_idx[q++] = r;
+    }
+    S.lam_start[5] = q;
+    S.km_nclass = 0;
+    for (int k = 0; k < 4; ++k)
+        if (used[k]) S.km_class[S.km_nclass++] = k;
+    std::memcpy(S.km_ma, d.km_ma, sizeof(S.km_ma));
+    std::memcpy(S.km_mb, d.km_mb, sizeof(S.km_mb));
+    std::memcpy(S.km_hw, d.km_hw, sizeof(S.km_hw));
+    std::memcpy(S.km_cplx, d.km_cplx, sizeof(S.km_cplx));
+    std::memcpy(S.km_shift_ma, d.km_shift_ma, sizeof(S.km_shift_ma));
+    std::memcpy(S.km_shift_mb, d.km_shift_mb, sizeof(S.km_shift_mb));
+    std::memcpy(S.km_shift_hw, d.km_shift_hw, sizeof(S.km_shift_hw));
+    S.mres[0] = d.mres[0];
+    S.mres[1] = d.mres[1];
+    S.mres[2] = d.mres[2];
+    S.scheme_nbins[0] = 1;
+    S.scheme_z[0][0] = 0.0;
+    for (int s = 0; s < d.n_schemes; ++s) {
+        S.scheme_nbins[s + 1] = d.scheme_nbins[s];
+        for (int b = 0; b < d.scheme_nbins[s]; ++b) S.scheme_z[s + 1][b] = d.scheme_z[s][b];
+    }
+}
+
+// Per-evaluation parameters: masses/widths, K-matrix blocks (the g_alpha /
+// f_scat packing of ConfigureCouplings and cu_init_data,
+// src/RooDalitzAmplitude.cxx:2831-2916, 3175-3190), pole lists.
+void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
+    const pwa_model_desc& d = c.desc;
+    std::memset(&D, 0, sizeof(D));
+    int nfs[4] = {0, 0, 0, 0};
+    for (int r = 0; r < c.nres; ++r) {
+        const double* ires = par + param_offset(d, r);
+        const int size = param_size(d, r);
+        D.m0[r] = ires[size - 5];
+        D.g0[r] = ires[size - 4];
+        if (d.type[r] != PWA_KMATRIX) continue;
+        const int ik = c.hs.ik[r];
+        if (D.m0[r] > 0) {
+            const int p = D.km_npole[ik];
+            if (p < PWA_KM_NPOLE) {
+                for (int ch = 0; ch < PWA_KM_NCH; ++ch) D.km[ik][p * PWA_KM_NCH + ch] = ires[8 + ch];
+                D.km_pole_res[ik][p] = r;
+                D.km_pole_shift[ik][p] = std::min(std::max((int)D.g0[r], 0), PWA_KM_NCH - 1);
+                D.km_npole[ik] = p + 1;
+            }
+        } else {
+            const int i = nfs[ik]++;
+            if (i < PWA_KM_NCH) {
+                double* f = &D.km[ik][16];
+                f[i] = ires[8];
+                f[i * PWA_KM_NCH] = ires[8];
+                if ((d.flags & PWA_F_KM_FULL_F) && i >= 1)
+                    for (int j = i; j < PWA_KM_NCH; ++j) {
+                        f[i * PWA_KM_NCH + j] = ires[9 + (j - i)];
+                        f[j * PWA_KM_NCH + i] = ires[9 + (j - i)];
+                    }
+            }
+        }
+    }
+}
+
+int ensure_partials(pwa_ctx* c) {
+    int64_t need = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int64_t cnt : c->smp[k].seg_count) need += (cnt + kBlock - 1) / kBlock;
+    need = std::max<int64_t>(need, 1);
+    if (need <= c->part_cap) return PWA_OK;
+    if (c->part) (void)hipFree(c->part);
+    c->part = nullptr;
+    HIPCHK(c, hipMalloc(&c->part, need * sizeof(Partial)));
+    c->part_cap = need;
+    return PWA_OK;
+}
+
+// Launch all segment kernels and the final reduction; returns the four
+// compensated sums per kind in c->hout (data: s0,c0,s1,c1; MC: s0,c0,s1,c1).
+int upload_params(pwa_ctx* c, const double* params) {
+    // the previous evaluation has synchronised the stream, so the pinned
+    // staging buffers are free to overwrite
+    build_dyn(*c, params, *c->hdyn);
+    std::vector<cplx> cv(c->ncpl);
+    c->cmap.eval(params, cv.data());
+    for (int j = 0; j < c->ncpl; ++j) c->hcpl[j] = C64(cv[j].real(), cv[j].imag());
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->dD, c->hdyn, sizeof(DevDyn), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->dC, c->hcpl, sizeof(C64) * std::max(c->ncpl, 1), hipMemcpyHostToDevice, c->st));
+    return PWA_OK;
+}
+
+int run_eval(pwa_ctx* c, const double* params, bool want_fx, bool force_unsmeared, bool mc_as_data) {
+    if (!c->smp[PWA_DATA].present && !c->smp[PWA_MC].present) return fail(c, PWA_E_STATE, "no events uploaded");
+    int rc = upload_params(c, params);
+    if (rc) return rc;
+    rc = ensure_partials(c);
+    if (rc) return rc;
+    const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
+    int32_t beg[3] = {0, 0, 0};
+    int64_t pb = 0;
+    HIPCHK(c, hipEventRecord(c->e0, c->st));
+    for (int k = 0; k < 2; ++k) {
+        Sample& s = c->smp[k];
+        beg[k] = (int32_t)pb;
+        if (!s.present) continue;
+        for (size_t q = 0; q < s.seg_scheme.size(); ++q) {
+            const int64_t n = s.seg_count[q];
+            if (n == 0) continue;
+            const int64_t off = s.seg_begin[q];
+            const int scheme = force_unsmeared ? 0 : s.seg_scheme[q];
+            const bool mc = (k == PWA_MC) && !mc_as_data;
+            HIPCHK(c, launch_nll(c->desc.n_pc, mc, scheme, s.ev + off, s.ld, n, c->dS, c->dD, c->dC,
+                                 want_fx ? s.fx + off : nullptr, c->part + pb, noint, c->st));
+            pb += (n + kBlock - 1) / kBlock;
+        }
+    }
+    beg[2] = (int32_t)pb;
+    HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, nullptr, c->dout, c->st));
+    HIPCHK(c, hipEventRecord(c->e1, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
+    return PWA_OK;
+}
+
+int copy_fx(pwa_ctx* c, int kind, double* out) {
+    Sample& s = c->smp[kind];
+    std::vector<double> tmp(s.n);
+    HIPCHK(c, hipMemcpy(tmp.data(), s.fx, s.n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t q = 0; q < s.n; ++q) out[s.perm[q]] = tmp[q];
+    return PWA_OK;
+}
+
+double nll_from_sums(const pwa_ctx* c, double* integral) {
+    const double* o = c->hout;
+    const double data_sum = o[0] + o[1];
+    const double I = o[4] + o[5];
+    const double Inoint = o[6] + o[7];
+    if (integral) *integral = I;
+    const Sample& d = c->smp[PWA_DATA];
+    const Sample& m = c->smp[PWA_MC];
+    double v = c->desc.nll_shift;
+    if (m.present && d.present) v += d.sumw * std::log(I / m.sumw);
+    v -= data_sum;
+    if ((c->desc.flags & PWA_F_NOINT_PENALTY) && std::fabs(Inoint / I - 1.0) > 0.3) v += 1000.0;
+    return v;
+}
+
+} // namespace
+
+// ======================================================================
+extern "C" {
+
+int pwa_fit_param_count(const pwa_model_desc* m, int32_t* n) {
+    if (!m || !n) return PWA_E_ARG;
+    *n = m->n_lambda * PWA_NPAR_LAMBDA + m->n_pc * PWA_NPAR_PC + m->n_zcs * PWA_NPAR_ZCS;
+    return PWA_OK;
+}
+
+int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
+    if (!model || !out) return PWA_E_ARG;
+    *out = nullptr;
+    std::string err;
+    int rc = validate(*model, err);
+    if (rc) return rc;
+    pwa_ctx* c = new pwa_ctx();
+    c->desc = *model;
+    c->device = device;
+    c->nres = model->n_lambda + model->n_pc + model->n_zcs;
+    if (!c->cmap.build(*model, err)) {
+        delete c;
+        return PWA_E_MODEL;
+    }
+    c->ncpl = (int)c->cmap.c.size();
+    build_static(*model, c->cmap, c->hs);
+    auto bad = [&](hipError_t e) {
+        if (e == hipSuccess) return false;
+        pwa_destroy(c);
+        return true;
+    };
+    if (bad(hipSetDevice(device))) return PWA_E_HIP;
+    if (bad(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking))) return PWA_E_HIP;
+    if (bad(hipMalloc(&c->dS, sizeof(DevStatic)))) return PWA_E_NOMEM;
+    if (bad(hipMalloc(&c->dD, sizeof(DevDyn)))) return PWA_E_NOMEM;
+    if (bad(hipMalloc(&c->dC, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
+    if (bad(hipMalloc(&c->dbeg, 4 * sizeof(int32_t)))) return PWA_E_NOMEM;
+    if (bad(hipMalloc(&c->dout, 8 * sizeof(double)))) return PWA_E_NOMEM;
+    c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams;
+    if (bad(hipMalloc(&c->dgout, 2 * c->NG * sizeof(double)))) return PWA_E_NOMEM;
+    if (bad(hipHostMalloc(&c->hgout, 2 * c->NG * sizeof(double)))) return PWA_E_NOMEM;
+    if (bad(hipHostMalloc(&c->hout, 8 * sizeof(double)))) return PWA_E_NOMEM;
+    if (bad(hipHostMalloc(&c->hdyn, sizeof(DevDyn)))) return PWA_E_NOMEM;
+    if (bad(hipHostMalloc(&c->hcpl, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
+    if (bad(hipEventCreate(&c->e0))) return PWA_E_HIP;
+    if (bad(hipEventCreate(&c->e1))) return PWA_E_HIP;
+    if (bad(hipMemcpy(c->dS, &c->hs, sizeof(DevStatic), hipMemcpyHostToDevice))) return PWA_E_HIP;
+    *out = c;
+    return PWA_OK;
+}
+
+int pwa_destroy(pwa_ctx* c) {
+    if (!c) return PWA_E_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (int k = 0; k < 2; ++k) {
+        if (c->smp[k].ev) (void)hipFree(c->smp[k].ev);
+        if (c->smp[k].fx) (void)hipFree(c->smp[k].fx);
+    }
+    if (c->dS) hipFree(c->dS);
+    if (c->dD) hipFree(c->dD);
+    if (c->dC) hipFree(c->dC);
+    if (c->part) (void)hipFree(c->part);
+    if (c->dbeg) (void)hipFree(c->dbeg);
+    if (c->dout) (void)hipFree(c->dout);
+    if (c->gpart) (void)hipFree(c->gpart);
+    if (c->dgout) (void)hipFree(c->dgout);
+    if (c->hgout) (void)hipHostFree(c->hgout);
+    if (c->hout) (void)hipHostFree(c->hout);
+    if (c->hdyn) (void)hipHostFree(c->hdyn);
+    if (c->hcpl) (void)hipHostFree(c->hcpl);
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return PWA_OK;
+}
+
+const char* pwa_last_error(const pwa_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pwa_upload_events(pwa_ctx* c, int kind, int64_t n, const double* cols, int64_t ld, const double* weight,
+                      const int32_t* scheme) {
+    if (!c) return PWA_E_ARG;
+    if (kind != PWA_DATA && kind != PWA_MC) return fail(c, PWA_E_ARG, "kind must be PWA_DATA or PWA_MC");
+    if (n < 0 || (n > 0 && (!cols || !weight || ld < n))) return fail(c, PWA_E_ARG, "bad event arrays");
+    if (n > (int64_t)1 << 31) return fail(c, PWA_E_ARG, "too many events");
+    for (int64_t i = 0; scheme && i < n; ++i)
+        if (scheme[i] < 0 || scheme[i] > c->desc.n_schemes)
+            return fail(c, PWA_E_ARG, "scheme id out of range at event " + std::to_string(i));
+    HIPCHK(c, hipSetDevice(c->device));
+    Sample& s = c->smp[kind];
+    if (s.ev) (void)hipFree(s.ev);
+    if (s.fx) (void)hipFree(s.fx);
+    s = Sample();
+    s.n = n;
+    s.ld = std::max<int64_t>(((n + 255) / 256) * 256, 256);
+    // order by scheme (stable): every segment shares one smearing scheme
+    s.perm.resize(n);
+    std::iota(s.perm.begin(), s.perm.end(), 0);
+    if (scheme)
+        std::stable_sort(s.perm.begin(), s.perm.end(), [&](int64_t a, int64_t b) { return scheme[a] < scheme[b]; });
+    for (int64_t q = 0; q < n; ++q) {
+        const int sc = scheme ? scheme[s.perm[q]] : 0;
+        if (s.seg_scheme.empty() || s.seg_scheme.back() != sc) {
+            s.seg_scheme.push_back(sc);
+            s.seg_begin.push_back(q);
+            s.seg_count.push_back(0);
+        }
+        s.seg_count.back()++;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        s.sumw += weight[i];
+        s.sumw2 += weight[i] * weight[i];
+    }
+    // raw columns in device order (PWA_NCOL columns + weight)
+    const int ncol = PWA_NCOL + 1;
+    std::vector<double> raw((size_t)ncol * std::max<int64_t>(n, 1));
+    for (int col = 0; col < PWA_NCOL; ++col)
+        for (int64_t q = 0; q < n; ++q) raw[(size_t)col * n + q] = cols[(size_t)col * ld + s.perm[q]];
+    for (int64_t q = 0; q < n; ++q) raw[(size_t)PWA_NCOL * n + q] = weight[s.perm[q]];
+    double* draw = nullptr;
+    HIPCHK(c, hipMalloc(&s.ev, (size_t)E_NPLANE * s.ld * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.fx, (size_t)s.ld * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(s.ev, 0, (size_t)E_NPLANE * s.ld * sizeof(double), c->st));
+    if (n > 0) {
+        HIPCHK(c, hipMalloc(&draw, raw.size() * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(draw, raw.data(), raw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, launch_prep(draw, n, n, s.ev, s.ld, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (draw) (void)hipFree(draw);
+    s.present = true;
+    return PWA_OK;
+}
+
+int pwa_eval(pwa_ctx* c, const double* params, double* nll, double* integral, double* fx_data, double* fx_mc) {
+    if (!c || !params) return PWA_E_ARG;
+    if (!c->smp[PWA_DATA].present || !c->smp[PWA_MC].present)
+        return fail(c, PWA_E_STATE, "both data and MC must be uploaded before pwa_eval");
+    const bool want = fx_data || fx_mc;
+    int rc = run_eval(c, params, want, false, false);
+    if (rc) return rc;
+    const double v = nll_from_sums(c, integral);
+    if (nll) *nll = v;
+    if (fx_data && (rc = copy_fx(c, PWA_DATA, fx_data))) return rc;
+    if (fx_mc && (rc = copy_fx(c, PWA_MC, fx_mc))) return rc;
+    return PWA_OK;
+}
+
+int pwa_amp2(pwa_ctx* c, int kind, const double* params, double* amp2) {
+    if (!c || !params || !amp2) return PWA_E_ARG;
+    if (kind != PWA_DATA && kind != PWA_MC) return fail(c, PWA_E_ARG, "bad kind");
+    if (!c->smp[kind].present) return fail(c, PWA_E_STATE, "sample not uploaded");
+    int rc = run_eval(c, params, true, true, true);
+    if (rc) return rc;
+    return copy_fx(c, kind, amp2);
+}
+
+int pwa_alpha(pwa_ctx* c, double* alpha) {
+    if (!c || !alpha) return PWA_E_ARG;
+    const Sample& d = c->smp[PWA_DATA];
+    if (!d.present || d.sumw2 <= 0) return fail(c, PWA_E_STATE, "no data weights");
+    *alpha = d.sumw / d.sumw2;
+    return PWA_OK;
+}
+
+int pwa_set_constraints(pwa_ctx* c, int32_t n, const int32_t* idx, const double* mu1, const double* mu2,
+                        const double* s1, const double* s2) {
+    if (!c || n < 0 || (n > 0 && (!idx || !mu1 || !mu2 || !s1 || !s2))) return PWA_E_ARG;
+    int32_t np = 0;
+    pwa_fit_param_count(&c->desc, &np);
+    c->cidx.assign(idx, idx + n);
+    for (int i = 0; i < n; ++i)
+        if (idx[i] < 0 || idx[i] >= np) return fail(c, PWA_E_ARG, "constraint index out of range");
+    c->cmu1.assign(mu1, mu1 + n);
+    c->cmu2.assign(mu2, mu2 + n);
+    c->cs1.assign(s1, s1 + n);
+    c->cs2.assign(s2, s2 + n);
+    return PWA_OK;
+}
+
+int pwa_eval_grad(pwa_ctx* c, const double* params, double* nll, double* grad) {
+    if (!c || !params || !grad) return PWA_E_ARG;
+    if (!c->smp[PWA_DATA].present || !c->smp[PWA_MC].present)
+        return fail(c, PWA_E_STATE, "both data and MC must be uploaded before pwa_eval_grad");
+    int rc = upload_params(c, params);
+    if (rc) return rc;
+    const int NG = c->NG;
+    const int kMaxBlocks = 512;
+    // rows: one per grad block, data segments first
+    int64_t rows = 0;
+    for (int k = 0; k < 2; ++k)
+        for (int64_t cnt : c->smp[k].seg_count)
+            if (cnt > 0) rows += std::min<int64_t>((cnt + kBlock - 1) / kBlock, kMaxBlocks);
+    rows = std::max<int64_t>(rows, 1);
+    if (rows > c->part_cap) {
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        HIPCHK(c, hipMalloc(&c->part, rows * sizeof(Partial)));
+        c->part_cap = rows;
+    }
+    if (rows * NG > c->gpart_cap) {
+        if (c->gpart) (void)hipFree(c->gpart);
+        c->gpart = nullptr;
+        HIPCHK(c, hipMalloc(&c->gpart, rows * NG * sizeof(double)));
+        c->gpart_cap = rows * NG;
+    }
+    const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
+    int32_t beg[3] = {0, 0, 0};
+    int64_t row = 0;
+    HIPCHK(c, hipEventRecord(c->e0, c->st));
+    for (int k = 0; k < 2; ++k) {
+        Sample& s = c->smp[k];
+        beg[k] = (int32_t)row;
+        for (size_t q = 0; q < s.seg_scheme.size(); ++q) {
+            const int64_t n = s.seg_count[q];
+            if (n == 0) continue;
+            const int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxBlocks);
+            HIPCHK(c, launch_grad(c->desc.n_pc, k == PWA_MC, s.seg_scheme[q], s.ev + s.seg_begin[q], s.ld, n, c->dS,
+                                  c->dD, c->dC, c->ncpl, c->nres, NG, blocks, c->gpart + row * NG, c->part + row,
+                                  noint, c->st));
+            row += blocks;
+        }
+    }
+    beg[2] = (int32_t)row;
+    HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, nullptr, c->dout, c->st));
+    HIPCHK(c, launch_grad_reduce(c->gpart, c->dbeg, NG, c->dgout, c->st));
+    HIPCHK(c, hipEventRecord(c->e1, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, 2 * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
+    double I = 0.0;
+    const double v = nll_from_sums(c, &I);
+    if (nll) *nll = v;
+    // kernel-level gradient: sumW_data * G_mc / I - G_data
+    const double* Gd = c->hgout;
+    const double* Gm = c->hgout + NG;
+    const double sc = c->smp[PWA_DATA].sumw / I;
+    std::vector<double> gk(NG);
+    for (int q = 0; q < NG; ++q) gk[q] = sc * Gm[q] - Gd[q];
+    // map to the fit-parameter vector
+    const pwa_model_desc& d = c->desc;
+    int32_t np = 0;
+    pwa_fit_param_count(&d, &np);
+    std::fill(grad, grad + np, 0.0);
+    std::vector<cplx> gc(c->ncpl);
+    for (int j = 0; j < c->ncpl; ++j) gc[j] = cplx(gk[2 * j], gk[2 * j + 1]);
+    c->cmap.chain(params, gc.data(), grad);
+    const int o_res = 2 * c->ncpl;
+    const int o_km = o_res + 2 * c->nres;
+    int npole[4] = {0, 0, 0, 0}, nnr[4] = {0, 0, 0, 0};
+    for (int r = 0; r < c->nres; ++r) {
+        const int off = param_offset(d, r), size = param_size(d, r);
+        grad[off + size - 5] += gk[o_res + 2 * r];
+        if (d.type[r] != PWA_KMATRIX) {
+            grad[off + size - 4] += gk[o_res + 2 * r + 1];
+            continue;
+        }
+        const int ik = c->hs.ik[r];
+        const double* gkm = &gk[o_km + kGradNClassParams * ik];
+        if (params[off + size - 5] > 0) {
+            const int p = npole[ik]++;
+            if (p >= PWA_KM_NPOLE) continue;
+            for (int ch = 0; ch < PWA_KM_NCH; ++ch) grad[off + 8 + ch] += gkm[p * PWA_KM_NCH + ch];
+            grad[off + size - 5] += gkm[32 + p];
+        } else {
+            const int i = nnr[ik]++;
+            if (i >= PWA_KM_NCH) continue;
+            grad[off + 8] += gkm[16 + i];
+            if ((d.flags & PWA_F_KM_FULL_F) && i >= 1)
+                for (int j = i; j < PWA_KM_NCH; ++j) grad[off + 9 + (j - i)] += gkm[16 + i * PWA_KM_NCH + j];
+        }
+    }
+    return PWA_OK;
+}
+
+int pwa_fcn(pwa_ctx* c, const double* params, double* fcn, double* grad) {
+    if (!c || !params || !fcn) return PWA_E_ARG;
+    double alpha = 0;
+    int rc = pwa_alpha(c, &alpha);
+    if (rc) return rc;
+    double nll = 0;
+    if (grad) rc = pwa_eval_grad(c, params, &nll, grad);
+    else rc = pwa_eval(c, params, &nll, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    int32_t np = 0;
+    pwa_fit_param_count(&c->desc, &np);
+    if (grad)
+        for (int i = 0; i < np; ++i) grad[i] *= alpha;
+    double pen = 0;
+    for (size_t q = 0; q < c->cidx.size(); ++q) {
+        const double x = params[c->cidx[q]];
+        double d = 0, s = 1;
+        if (x < c->cmu1[q]) { d = x - c->cmu1[q]; s = c->cs1[q]; }
+        else if (x > c->cmu2[q]) { d = x - c->cmu2[q]; s = c->cs2[q]; }
+        pen += 0.5 * (d / s) * (d / s);
+        if (grad) grad[c->cidx[q]] += d / (s * s);
+    }
+    *fcn = (nll - c->desc.nll_shift) * alpha + pen;
+    return PWA_OK;
+}
+
+int pwa_helicity_couplings(const pwa_model_desc* model, const double* params, double* out, int32_t* n) {
+    if (!model || !n) return PWA_E_ARG;
+    std::string err;
+    if (validate(*model, err)) return PWA_E_MODEL;
+    CouplingMap cm;
+    if (!cm.build(*model, err)) return PWA_E_MODEL;
+    *n = (int32_t)cm.c.size();
+    if (out) {
+        if (!params) return PWA_E_ARG;
+        std::vector<cplx> v(cm.c.size());
+        cm.eval(params, v.data());
+        for (size_t j = 0; j < v.size(); ++j) {
+            out[2 * j] = v[j].real();
+            out[2 * j + 1] = v[j].imag();
+        }
+    }
+    return PWA_OK;
+}
+
+int pwa_last_timing(pwa_ctx* c, double* ms_total, double* ms_main) {
+    if (!c) return PWA_E_ARG;
+    if (ms_total) *ms_total = c->last_ms;
+    if (ms_main) *ms_main = c->last_ms;
+    return PWA_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,272 @@
+// Synthetic Lambda_b -> J/psi p K-, J/psi -> mu+ mu- events and their
+// helicity variables.
+//
+// Generator: uniform three-body phase space (flat in the Dalitz plane),
+// isotropic J/psi decay, the Lambda_b boosted with beta ~ U(0.7, 0.9) in a
+// direction with theta ~ U(0, pi), phi ~ U(-pi, pi) -- the recipe of
+// gen_toys/phsp_toys/gentoy_formhj.C:10-115.  Angles: the conventions of
+// gen_toys/phsp_toys/tree.C:127-408 (helicityJpsiLam, helicityZK,
+// helicityTwoFrame) for the Lambda* and Pc chains.  The reference defines no
+// Zcs-chain angles; they are the Pc-chain definitions with the proton and
+// kaon exchanged (X = J/psi K-, recoil p).
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "pwa_amp.h"
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double mag(V3 a) { return std::sqrt(dot(a, a)); }
+V3 unit(V3 a) {
+    const double m = mag(a);
+    return m > 0 ? (1.0 / m) * a : a;
+}
+
+struct P4 {
+    V3 p;
+    double e;
+    double m() const { return std::sqrt(std::fmax(0.0, e * e - dot(p, p))); }
+    V3 beta() const { return (1.0 / e) * p; }
+};
+P4 operator+(P4 a, P4 b) { return {a.p + b.p, a.e + b.e}; }
+
+// Pure boost by velocity b (the TLorentzVector::Boost convention).
+P4 boost(P4 v, V3 b) {
+    const double b2 = dot(b, b);
+    if (b2 <= 0) return v;
+    const double g = 1.0 / std::sqrt(1.0 - b2);
+    const double bp = dot(b, v.p);
+    const double g2 = (g - 1.0) / b2;
+    return {v.p + (g2 * bp + g * v.e) * b, g * (v.e + bp)};
+}
+
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+        for (int i = 0; i < 4; ++i) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t x = z;
+            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+            s[i] = x ^ (x >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {  // xoshiro256**
+        const uint64_t r = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uni() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    double uni(double a, double b) { return a + (b - a) * uni(); }
+};
+
+double kallen_p(double M, double m1, double m2) {
+    const double a = (M * M - (m1 + m2) * (m1 + m2)) * (M * M - (m1 - m2) * (m1 - m2));
+    return std::sqrt(std::fmax(a, 0.0)) / (2.0 * M);
+}
+
+V3 random_dir(Rng& r) {
+    const double c = r.uni(-1.0, 1.0), ph = r.uni(-M_PI, M_PI);
+    const double s = std::sqrt(std::fmax(0.0, 1.0 - c * c));
+    return {s * std::cos(ph), s * std::sin(ph), c};
+}
+
+// Two-body decay of a particle at rest into (m1 along +dir, m2 along -dir).
+void two_body(double M, double m1, double m2, V3 dir, P4& a, P4& b) {
+    const double p = kallen_p(M, m1, m2);
+    a = {p * dir, std::sqrt(p * p + m1 * m1)};
+    b = {-(p * dir), std::sqrt(p * p + m2 * m2)};
+}
+
+// Helicity variables of one event from the lab four-vectors
+// (tree.C:127-408; Zcs chain by exchanging the proton and the kaon).
+void angles(const P4& mup, const P4& mum, const P4& pr, const P4& ka, double* o) {
+    const P4 psi = mup + mum, lam = pr + ka, B = psi + lam;
+    o[PWA_COL_MKP] = lam.m();
+    o[PWA_COL_MJPSIP] = (psi + pr).m();
+    o[PWA_COL_MJPSIK] = (psi + ka).m();
+    const V3 PBunit = unit(B.p);
+    const V3 bB = -B.beta();
+    // ---- Lambda* chain (helicityJpsiLam)
+    {
+        P4 mu2 = boost(mum, bB), p = boost(pr, bB), J = boost(psi, bB), L = boost(lam, bB);
+        const V3 LUnit = unit(L.p), psiUnit = unit(J.p);
+        o[PWA_COL_COS_LB] = dot(PBunit, LUnit);
+        p = boost(p, -L.beta());
+        const V3 Unit = unit(p.p);
+        o[PWA_COL_COS_L] = -dot(psiUnit, Unit);
+        V3 aboost = -unit(B.p - dot(B.p, LUnit) * LUnit);
+        o[PWA_COL_PHI_K] = std::atan2(dot(unit(cross(-psiUnit, aboost)), Unit), dot(aboost, Unit));
+        aboost = -aboost;
+        mu2 = boost(mu2, -J.beta());
+        const V3 U2 = unit(mu2.p);
+        o[PWA_COL_COS_JPSI] = dot(psiUnit, U2);
+        o[PWA_COL_PHI_MU] = std::atan2(dot(unit(cross(psiUnit, aboost)), U2), dot(aboost, U2));
+    }
+    // ---- two-body chains (helicityZK): R = J/psi + b, recoil c
+    auto chain = [&](const P4& b, const P4& cc, int cLb, int cZ, int cJ, int pZ, int pJ, int pM) {
+        const P4 R = psi + b;
+        P4 mu2 = boost(mum, bB), bb = boost(b, bB), k = boost(cc, bB), J = boost(psi, bB), L = boost(lam, bB),
+           Z = boost(R, bB);
+        (void)bb;
+        const V3 ZUnit = unit(Z.p);
+        o[cLb] = dot(PBunit, ZUnit);
+        // the Lambda* momentum fixes the azimuth origin (tree.C:259)
+        V3 aboost = unit(L.p - dot(L.p, PBunit) * PBunit);
+        o[pZ] = std::atan2(dot(unit(cross(PBunit, aboost)), ZUnit), dot(aboost, ZUnit));
+        const V3 bZ = -Z.beta();
+        J = boost(J, bZ);
+        mu2 = boost(mu2, bZ);
+        k = boost(k, bZ);
+        const V3 Unit = unit(J.p);
+        o[cZ] = -dot(unit(k.p), Unit);
+        aboost = -unit(B.p - dot(B.p, ZUnit) * ZUnit);
+        o[pJ] = std::atan2(dot(unit(cross(-unit(k.p), aboost)), Unit), dot(aboost, Unit));
+        const V3 psiUnit = unit(J.p);
+        mu2 = boost(mu2, -J.beta());
+        const V3 U2 = unit(mu2.p);
+        o[cJ] = dot(psiUnit, U2);
+        aboost = -unit(-k.p + dot(k.p, psiUnit) * psiUnit);
+        o[pM] = std::atan2(dot(unit(cross(psiUnit, aboost)), U2), dot(aboost, U2));
+    };
+    chain(pr, ka, PWA_COL_Z_COS_LB, PWA_COL_Z_COS_Z, PWA_COL_Z_COS_JPSI, PWA_COL_Z_PHI_Z, PWA_COL_Z_PHI_JPSI,
+          PWA_COL_Z_PHI_MU);
+    chain(ka, pr, PWA_COL_X_COS_LB, PWA_COL_X_COS_X, PWA_COL_X_COS_JPSI, PWA_COL_X_PHI_X, PWA_COL_X_PHI_JPSI,
+          PWA_COL_X_PHI_MU);
+    // ---- alignment angles (helicityTwoFrame)
+    auto align = [&](const P4& b, const P4& cc, int cP, int aM) {
+        const P4 R = psi + b;
+        P4 mu2 = boost(mum, bB), bb = boost(b, bB), k = boost(cc, bB), J = boost(psi, bB), L = boost(lam, bB),
+           Z = boost(R, bB);
+        P4 kb = boost(k, -bb.beta()), Jb = boost(J, -bb.beta());
+        o[cP] = dot(unit(kb.p), unit(Jb.p));
+        mu2 = boost(mu2, -J.beta());
+        const P4 bJ = boost(bb, -J.beta());
+        const P4 LJ = boost(L, -J.beta());
+        const V3 z3 = unit(mu2.p);
+        const V3 x3Z = -unit(-bJ.p + dot(bJ.p, z3) * z3);
+        const V3 x3L = -unit(-LJ.p + dot(LJ.p, z3) * z3);
+        o[aM] = std::atan2(dot(unit(cross(z3, x3Z)), x3L), dot(x3Z, x3L));
+        (void)Z;
+    };
+    align(pr, ka, PWA_COL_COS_P, PWA_COL_ALPHA_MU);
+    align(ka, pr, PWA_COL_X_COS_P, PWA_COL_X_ALPHA_MU);
+}
+
+} // namespace
+
+extern "C" int pwa_phsp_angles(int64_t n, const double* p4, double* cols) {
+    // p4: per event 16 doubles (mu+, mu-, p, K-; each px, py, pz, E)
+    if (n < 0 || (n > 0 && (!p4 || !cols))) return PWA_E_ARG;
+    double o[PWA_NCOL];
+    for (int64_t i = 0; i < n; ++i) {
+        const double* q = p4 + 16 * i;
+        P4 v[4];
+        for (int a = 0; a < 4; ++a) v[a] = {{q[4 * a], q[4 * a + 1], q[4 * a + 2]}, q[4 * a + 3]};
+        for (int c = 0; c < PWA_NCOL; ++c) o[c] = 0.0;
+        angles(v[0], v[1], v[2], v[3], o);
+        for (int c = 0; c < PWA_NCOL; ++c) cols[(int64_t)c * n + i] = o[c];
+    }
+    return PWA_OK;
+}
+
+extern "C" int pwa_generate_phsp_p4(uint64_t seed, int64_t n, double m_lb, double m_jpsi, double m_p, double m_k,
+                                    double m_mu, double* p4) {
+    if (n < 0 || (n > 0 && !p4)) return PWA_E_ARG;
+    if (m_jpsi + m_p + m_k >= m_lb || 2 * m_mu >= m_jpsi) return PWA_E_ARG;
+    Rng r(seed);
+    const double s12lo = (m_p + m_k) * (m_p + m_k), s12hi = (m_lb - m_jpsi) * (m_lb - m_jpsi);
+    const double s23lo = (m_jpsi + m_p) * (m_jpsi + m_p), s23hi = (m_lb - m_k) * (m_lb - m_k);
+    for (int64_t i = 0; i < n; ++i) {
+        // flat Dalitz plane in (m^2(pK), m^2(J/psi p)) by rejection
+        double mpk, mjp;
+        for (;;) {
+            const double s12 = r.uni(s12lo, s12hi), s23 = r.uni(s23lo, s23hi);
+            mpk = std::sqrt(s12);
+            // p energy and momentum in the (pK) rest frame; J/psi there
+            const double Ep = (s12 - m_k * m_k + m_p * m_p) / (2 * mpk);
+            const double Ej = (m_lb * m_lb - s12 - m_jpsi * m_jpsi) / (2 * mpk);
+            const double pp = std::sqrt(std::fmax(Ep * Ep - m_p * m_p, 0.0));
+            const double pj = std::sqrt(std::fmax(Ej * Ej - m_jpsi * m_jpsi, 0.0));
+            const double lo = (Ep + Ej) * (Ep + Ej) - (pp + pj) * (pp + pj);
+            const double hi = (Ep + Ej) * (Ep + Ej) - (pp - pj) * (pp - pj);
+            if (s23 >= lo && s23 <= hi) {
+                mjp = std::sqrt(s23);
+                break;
+            }
+        }
+        // Lambda_b rest frame: (pK) system and J/psi back to back
+        P4 Lam, Psi;
+        two_body(m_lb, mpk, m_jpsi, random_dir(r), Lam, Psi);
+        // p and K in the (pK) rest frame with the angle fixed by m(J/psi p)
+        const double Ep = (mpk * mpk - m_k * m_k + m_p * m_p) / (2 * mpk);
+        const double pp = std::sqrt(std::fmax(Ep * Ep - m_p * m_p, 0.0));
+        const double Ej = (m_lb * m_lb - mpk * mpk - m_jpsi * m_jpsi) / (2 * mpk);
+        const double pj = std::sqrt(std::fmax(Ej * Ej - m_jpsi * m_jpsi, 0.0));
+        double cth = (m_jpsi * m_jpsi + m_p * m_p + 2 * Ep * Ej - mjp * mjp) / (2 * pp * pj);
+        cth = std::fmin(1.0, std::fmax(-1.0, cth));
+        // angle between p and the J/psi direction in the (pK) frame; J/psi
+        // moves opposite to the Lambda_b recoil there, i.e. along -Lam direction
+        const V3 zj = unit(-Lam.p);
+        V3 t = std::fabs(zj.x) < 0.9 ? V3{1, 0, 0} : V3{0, 1, 0};
+        const V3 e1 = unit(cross(zj, t)), e2 = cross(zj, e1);
+        const double ph = r.uni(-M_PI, M_PI), sth = std::sqrt(std::fmax(0.0, 1 - cth * cth));
+        const V3 dp = cth * zj + (sth * std::cos(ph)) * e1 + (sth * std::sin(ph)) * e2;
+        P4 pr, ka;
+        two_body(mpk, m_p, m_k, dp, pr, ka);
+        pr = boost(pr, Lam.beta());
+        ka = boost(ka, Lam.beta());
+        // J/psi -> mu+ mu- isotropic
+        P4 mup, mum;
+        two_body(m_jpsi, m_mu, m_mu, random_dir(r), mup, mum);
+        mup = boost(mup, Psi.beta());
+        mum = boost(mum, Psi.beta());
+        // Lambda_b to the lab
+        const double beta = r.uni(0.7, 0.9), th = r.uni(0.0, M_PI), phb = r.uni(-M_PI, M_PI);
+        const V3 bv = beta * V3{std::sin(th) * std::cos(phb), std::sin(th) * std::sin(phb), std::cos(th)};
+        const P4 out[4] = {boost(mup, bv), boost(mum, bv), boost(pr, bv), boost(ka, bv)};
+        double* q = p4 + 16 * i;
+        for (int a = 0; a < 4; ++a) {
+            q[4 * a] = out[a].p.x;
+            q[4 * a + 1] = out[a].p.y;
+            q[4 * a + 2] = out[a].p.z;
+            q[4 * a + 3] = out[a].e;
+        }
+    }
+    return PWA_OK;
+}
+
+extern "C" int pwa_generate_phsp(uint64_t seed, int64_t n, double m_lb, double m_jpsi, double m_p, double m_k,
+                                 double m_mu, double* cols) {
+    if (n < 0 || (n > 0 && !cols)) return PWA_E_ARG;
+    const int64_t chunk = 1 << 16;
+    std::vector<double> p4((size_t)16 * std::min<int64_t>(n, chunk));
+    std::vector<double> tmp((size_t)PWA_NCOL * std::min<int64_t>(n, chunk));
+    for (int64_t b = 0; b < n; b += chunk) {
+        const int64_t m = std::min(chunk, n - b);
+        int rc = pwa_generate_phsp_p4(seed * 1000003ull + (uint64_t)(b / chunk), m, m_lb, m_jpsi, m_p, m_k, m_mu, p4.data());
+        if (rc) return rc;
+        pwa_phsp_angles(m, p4.data(), tmp.data());
+        for (int c = 0; c < PWA_NCOL; ++c)
+            for (int64_t i = 0; i < m; ++i) cols[(int64_t)c * n + b + i] = tmp[(int64_t)c * m + i];
+    }
+    return PWA_OK;
+}

@@ -1,0 +1,382 @@
+// Device math for the Lambda_b -> J/psi p K- amplitude: complex numbers, a
+// forward-mode dual number, Wigner small-d functions and the lineshapes
+// (Breit-Wigner with Blatt-Weisskopf barriers, Lambda_b-vertex barrier
+// terms, K-matrix).  Everything is templated on the real scalar type T so
+// the same code gives values (T = double) and parameter derivatives
+// (T = Dual<N>).  All arithmetic is IEEE FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PWA_HD __host__ __device__ __forceinline__
+
+namespace pwa {
+
+// ------------------------------------------------------------------ dual
+// Value plus N directional derivatives.
+template <int N>
+struct Dual {
+    double v;
+    double d[N];
+    PWA_HD Dual() : v(0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = 0;
+    }
+    PWA_HD Dual(double x) : v(x) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = 0;
+    }
+    PWA_HD static Dual var(double x, int i) {
+        Dual r(x);
+        r.d[i] = 1.0;
+        return r;
+    }
+};
+
+template <class T> PWA_HD double val(const T& x) { return x.v; }
+PWA_HD double val(const double& x) { return x; }
+
+#define PWA_DUAL_BINOP(OP, VEXPR, DEXPR)                                                        \
+    template <int N> PWA_HD Dual<N> operator OP(const Dual<N>& a, const Dual<N>& b) {           \
+        Dual<N> r;                                                                              \
+        r.v = VEXPR;                                                                            \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) r.d[i] = DEXPR;                           \
+        return r;                                                                               \
+    }
+PWA_DUAL_BINOP(+, a.v + b.v, a.d[i] + b.d[i])
+PWA_DUAL_BINOP(-, a.v - b.v, a.d[i] - b.d[i])
+PWA_DUAL_BINOP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+#undef PWA_DUAL_BINOP
+template <int N> PWA_HD Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r;
+    const double ib = 1.0 / b.v;
+    r.v = a.v * ib;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+}
+template <int N> PWA_HD Dual<N> operator+(const Dual<N>& a, double b) { Dual<N> r = a; r.v += b; return r; }
+template <int N> PWA_HD Dual<N> operator+(double b, const Dual<N>& a) { return a + b; }
+template <int N> PWA_HD Dual<N> operator-(const Dual<N>& a, double b) { Dual<N> r = a; r.v -= b; return r; }
+template <int N> PWA_HD Dual<N> operator-(double b, const Dual<N>& a) {
+    Dual<N> r;
+    r.v = b - a.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+    return r;
+}
+template <int N> PWA_HD Dual<N> operator-(const Dual<N>& a) { return 0.0 - a; }
+template <int N> PWA_HD Dual<N> operator*(const Dual<N>& a, double b) {
+    Dual<N> r;
+    r.v = a.v * b;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b;
+    return r;
+}
+template <int N> PWA_HD Dual<N> operator*(double b, const Dual<N>& a) { return a * b; }
+template <int N> PWA_HD Dual<N> operator/(const Dual<N>& a, double b) { return a * (1.0 / b); }
+template <int N> PWA_HD Dual<N> operator/(double a, const Dual<N>& b) {
+    Dual<N> r;
+    const double ib = 1.0 / b.v;
+    r.v = a * ib;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = -r.v * b.d[i] * ib;
+    return r;
+}
+template <int N> PWA_HD Dual<N> dsqrt(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = sqrt(a.v);
+    const double h = 0.5 / r.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+    return r;
+}
+template <int N> PWA_HD Dual<N> dtanh(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = tanh(a.v);
+    const double h = 1.0 - r.v * r.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+    return r;
+}
+template <int N> PWA_HD Dual<N> dfabs(const Dual<N>& a) { return a.v < 0 ? -a : a; }
+PWA_HD double dsqrt(double a) { return sqrt(a); }
+PWA_HD double dtanh(double a) { return tanh(a); }
+PWA_HD double dfabs(double a) { return fabs(a); }
+
+// x^n for a small non-negative integer n (exact multiplication chain).
+template <class T> PWA_HD T ipow(T x, int n) {
+    T r(1.0);
+    for (int i = 0; i < n; ++i) r = r * x;
+    return r;
+}
+
+// --------------------------------------------------------------- complex
+template <class T>
+struct Cx {
+    T re, im;
+    PWA_HD Cx() : re(0.0), im(0.0) {}
+    PWA_HD Cx(T a) : re(a), im(0.0) {}
+    PWA_HD Cx(T a, T b) : re(a), im(b) {}
+};
+using C64 = Cx<double>;
+
+template <class T> PWA_HD Cx<T> operator+(const Cx<T>& a, const Cx<T>& b) { return Cx<T>(a.re + b.re, a.im + b.im); }
+template <class T> PWA_HD Cx<T> operator-(const Cx<T>& a, const Cx<T>& b) { return Cx<T>(a.re - b.re, a.im - b.im); }
+template <class T> PWA_HD Cx<T> operator*(const Cx<T>& a, const Cx<T>& b) {
+    return Cx<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <class T> PWA_HD Cx<T> operator*(const Cx<T>& a, const T& s) { return Cx<T>(a.re * s, a.im * s); }
+template <class T> PWA_HD Cx<T> operator*(const T& s, const Cx<T>& a) { return Cx<T>(a.re * s, a.im * s); }
+template <int N> PWA_HD Cx<Dual<N>> operator*(const Cx<Dual<N>>& a, double s) { return Cx<Dual<N>>(a.re * s, a.im * s); }
+template <int N> PWA_HD Cx<Dual<N>> operator*(double s, const Cx<Dual<N>>& a) { return Cx<Dual<N>>(a.re * s, a.im * s); }
+template <class T> PWA_HD Cx<T> conj(const Cx<T>& a) { return Cx<T>(a.re, 0.0 - a.im); }
+template <class T> PWA_HD T norm2(const Cx<T>& a) { return a.re * a.re + a.im * a.im; }
+template <class T> PWA_HD Cx<T> cinv(const Cx<T>& a) {
+    T s = 1.0 / (a.re * a.re + a.im * a.im);
+    return Cx<T>(a.re * s, (0.0 - a.im) * s);
+}
+// a += b * c (complex multiply-add)
+PWA_HD void cmac(C64& a, const C64& b, const C64& c) {
+    a.re = fma(b.re, c.re, fma(-b.im, c.im, a.re));
+    a.im = fma(b.re, c.im, fma(b.im, c.re, a.im));
+}
+// a += s * b (real times complex)
+PWA_HD void rmac(C64& a, double s, const C64& b) {
+    a.re = fma(s, b.re, a.re);
+    a.im = fma(s, b.im, a.im);
+}
+
+// ------------------------------------------------------------ Wigner d
+// d^j_{m' m}(beta) from cos(beta/2) = c and sin(beta/2) = s by the explicit
+// Wigner sum (all spin arguments are twice the quantum number):
+//   d = sum_k (-1)^(m'-m+k) sqrt((j+m')!(j-m')!(j+m)!(j-m)!) /
+//       ((j+m-k)! k! (m'-m+k)! (j-m'-k)!) c^(2j+m-m'-2k) s^(m'-m+2k).
+// The exponents and coefficients are compile-time constants.
+constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+constexpr double csqrt_nr(double x, double g, int it) { return it == 0 ? g : csqrt_nr(x, 0.5 * (g + x / g), it - 1); }
+constexpr double csqrtc(double x) { return x <= 0 ? 0.0 : csqrt_nr(x, x > 1 ? x : 1.0, 200); }
+
+template <int J2, int MP2, int M2, int K>
+struct WignerTerm {
+    static constexpr int jmp = (J2 + MP2) / 2, jmm = (J2 - MP2) / 2, jpm = (J2 + M2) / 2, jm = (J2 - M2) / 2;
+    static constexpr int mpm = (MP2 - M2) / 2;
+    static constexpr bool valid = (jpm - K >= 0) && (mpm + K >= 0) && (jmm - K >= 0);
+    static constexpr int pc = J2 + (M2 - MP2) / 2 * 1 - 2 * K; // 2j + m - m' - 2k (j, m in units of 1/2: J2 = 2j)
+    static constexpr int ps = mpm + 2 * K;
+    static constexpr double coef() {
+        return valid ? (((mpm + K) % 2 == 0 ? 1.0 : -1.0) *
+                        csqrtc(cfact(jmp) * cfact(jmm) * cfact(jpm) * cfact(jm)) /
+                        (cfact(jpm - K) * cfact(K) * cfact(mpm + K) * cfact(jmm - K)))
+                     : 0.0;
+    }
+};
+
+template <int J2, int MP2, int M2, int K>
+PWA_HD double wigner_sum(const double* cp, const double* sp) {
+    if constexpr (K > J2) {
+        return 0.0;
+    } else {
+        using W = WignerTerm<J2, MP2, M2, K>;
+        double t = 0.0;
+        if constexpr (W::valid) {
+            constexpr double co = W::coef();
+            t = co * cp[W::pc] * sp[W::ps];
+        }
+        return t + wigner_sum<J2, MP2, M2, K + 1>(cp, sp);
+    }
+}
+
+// cp[n] = c^n, sp[n] = s^n for n <= 9 must be provided.
+template <int J2, int MP2, int M2>
+PWA_HD double wigner_d(const double* cp, const double* sp) {
+    static_assert((J2 + MP2) % 2 == 0 && (J2 + M2) % 2 == 0, "parity of spin labels");
+    if constexpr (MP2 > J2 || MP2 < -J2 || M2 > J2 || M2 < -J2) return 0.0;
+    else return wigner_sum<J2, MP2, M2, 0>(cp, sp);
+}
+
+PWA_HD void powers(double x, double* p, int n) {
+    p[0] = 1.0;
+    for (int i = 1; i <= n; ++i) p[i] = p[i - 1] * x;
+}
+
+// ------------------------------------------------------- kinematic helpers
+// Two-body breakup momentum |p| = sqrt(|(M^2-(m1+m2)^2)(M^2-(m1-m2)^2)|)/(2M).
+template <class T>
+PWA_HD T breakup(T M, double m1, double m2) {
+    T M2 = M * M;
+    return dsqrt(dfabs((M2 - (m1 + m2) * (m1 + m2)) * (M2 - (m1 - m2) * (m1 - m2)))) / (2.0 * M);
+}
+PWA_HD double breakup_d(double M, double m1, double m2) { return breakup<double>(M, m1, m2); }
+
+// Blatt-Weisskopf barrier ratio F_L(p)/F_L(p0) (von Hippel-Quigg polynomials),
+// returned as sqrt(B_L(z0)/B_L(z)) with z = (r p)^2.
+template <class T>
+PWA_HD T bw_poly(T z, int L) {
+    switch (L) {
+    case 1: return 1.0 + z;
+    case 2: return z * z + 3.0 * z + 9.0;
+    case 3: return z * z * z + 6.0 * z * z + 45.0 * z + 225.0;
+    case 4: return z * z * z * z + 10.0 * z * z * z + 135.0 * z * z + 1575.0 * z + 11025.0;
+    case 5: return z * z * z * z * z + 15.0 * z * z * z * z + 315.0 * z * z * z + 6300.0 * z * z + 99225.0 * z + 893025.0;
+    default: return T(1.0);
+    }
+}
+template <class T>
+PWA_HD T barrier_ratio(double r, int L, T p, T p0) {
+    if (L < 1 || L > 5) return T(1.0);
+    T z = (r * p) * (r * p);
+    T z0 = (r * p0) * (r * p0);
+    return dsqrt(bw_poly(z0, L) / bw_poly(z, L));
+}
+
+// Squared-barrier form for K-matrix phase space, with the reference radius
+// s0 = 25 GeV^2: (p^2/p0^2)^L * B_L(z0)/B_L(z), z = r^2 p^2, z0 = r^2 |p0^2|.
+PWA_HD double fbw2(double M2, double m1, double m2, int L, double r) {
+    if (L == 0) return 1.0;
+    double p2 = (M2 - (m1 + m2) * (m1 + m2)) * (M2 - (m1 - m2) * (m1 - m2)) / M2 / 4.0;
+    const double M20 = 25.0;
+    double p20 = (M20 - (m1 + m2) * (m1 + m2)) * (M20 - (m1 - m2) * (m1 - m2)) / M20 / 4.0;
+    double ratio = ipow(p2 / p20, L);
+    if (L > 5) return ratio;
+    double z = r * r * p2, z0 = r * r * fabs(p20);
+    return ratio * bw_poly(z0, L) / bw_poly(z, L);
+}
+
+// Square root with the branch rule of the reference's complex sqrt: zero
+// inside |x|,|y| < 1e-6; negative imaginary part only when Im < 0 and
+// |Im/Re| > 1e-3; otherwise the magnitudes of the principal root.
+PWA_HD C64 csqrt_branch(C64 x) {
+    if (fabs(x.re) < 0.000001 && fabs(x.im) < 0.000001) return C64(0.0, 0.0);
+    double radius = hypot(x.re, x.im);
+    double cosA = x.re / radius;
+    double re = sqrt(radius * (cosA + 1.0) / 2.0);
+    double im = sqrt(radius * (1.0 - cosA) / 2.0);
+    if (signbit(x.im) && fabs(x.im / x.re) > 0.001) im = -im;
+    return C64(re, im);
+}
+
+// ------------------------------------------------------------- lineshapes
+struct Masses {
+    double m_lb, m_p, m_k, m_jpsi, ffr, ffb;
+};
+
+// 1/(m^2 - s - i m Gamma)
+template <class T>
+PWA_HD Cx<T> bw_prop(T m, T gamma, double s) {
+    T de2 = m * m - s;
+    T gm = gamma * m;
+    T inv = 1.0 / (de2 * de2 + gm * gm);
+    return Cx<T>(de2 * inv, gm * inv);
+}
+
+// Normalisation 0.948850 * m * sqrt(Gamma * k / sqrt(m (m + k))), k = sqrt(m^2 + Gamma^2).
+template <class T>
+PWA_HD T bw_normalisation(T m, T g) {
+    T kr = dsqrt(m * m + g * g);
+    return 0.948850 * m * dsqrt(g * kr / dsqrt(m * (m + kr)));
+}
+
+// Effective mass mmin + (mmax-mmin)/2 * (1 + tanh((m - (mmin+mmax)/2)/(mmax-mmin))).
+template <class T>
+PWA_HD T tanh_mass(T m, double mmin, double mmax) {
+    return mmin + 0.5 * (mmax - mmin) * (1.0 + dtanh((m - (mmin + mmax) / 2.0) / (mmax - mmin)));
+}
+
+// Lambda* Breit-Wigner in m(K p), with the below-threshold effective mass and
+// the Sigma pi width for 0 < m_r < 1.43 GeV.
+template <class T>
+PWA_HD Cx<T> bw_lambda(const Masses& P, T m_r, T g_r, double mpp, int LR) {
+    const double thr = P.m_p + P.m_k;
+    const double pR = breakup_d(mpp, P.m_p, P.m_k);
+    T pR0(pR);
+    const double mr = val(m_r);
+    if (mr < -0.0) {
+        const double mmin = thr, mmax = P.m_lb - P.m_jpsi;
+        pR0 = T(breakup_d((mmin + mmax) / 2.0, P.m_p, P.m_k));
+    }
+    if (mr > thr) pR0 = breakup(m_r, P.m_p, P.m_k);
+    if (mr <= thr && mr > 1e-10) pR0 = breakup(tanh_mass(m_r, thr, P.m_lb - P.m_jpsi), P.m_p, P.m_k);
+    T FR = barrier_ratio(P.ffr, LR, T(pR), pR0);
+    if (mr < -0.0) return Cx<T>(ipow(pR / pR0, LR), T(0.0));
+    T gamma = g_r;
+    if (mr > thr) {
+        gamma = g_r * ipow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    } else if (mr < 1.43 && mr > 0) {
+        const double msig = 1.18937, mpi0 = 0.13957018;
+        const double pRSigPi = breakup_d(mpp, msig, mpi0);
+        pR0 = breakup(m_r, msig, mpi0);
+        FR = T(1.0);
+        gamma = g_r * ipow(pR / pR0, 2 * LR + 1) * (m_r / mpp) + 0.0505 * ipow(pRSigPi / pR0, 2 * LR + 1) * (m_r / mpp);
+    }
+    T factor = FR * ipow(pR / pR0, LR) * bw_normalisation(m_r, g_r);
+    return bw_prop(m_r, gamma, mpp * mpp) * factor;
+}
+
+// Breit-Wigner of a two-body system (ma, mb): the Pc (J/psi p) and Zcs (J/psi K) form.
+template <class T>
+PWA_HD Cx<T> bw_twobody(double ma, double mb, double ffr, T m_r, T g_r, double mpp, int LR) {
+    const double pR = breakup_d(mpp, ma, mb);
+    T pR0(pR);
+    const bool above = val(m_r) > ma + mb;
+    if (above) pR0 = breakup(m_r, ma, mb);
+    T FR = barrier_ratio(ffr, LR, T(pR), pR0);
+    T gamma = g_r;
+    if (above) gamma = g_r * ipow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    T factor = FR * ipow(pR / pR0, LR) * bw_normalisation(m_r, g_r);
+    return bw_prop(m_r, gamma, mpp * mpp) * factor;
+}
+
+// Breakup momentum of M -> m1 + m2 with a parameter-dependent daughter mass m2.
+template <class T>
+PWA_HD T breakup_dm(double M, double m1, T m2) {
+    const double M2 = M * M;
+    T a = M2 - (m1 + m2) * (m1 + m2);
+    T b = M2 - (m1 - m2) * (m1 - m2);
+    return dsqrt(dfabs(a * b)) / (2.0 * M);
+}
+
+// Lambda_b -> J/psi Lambda* vertex barrier term (orbital LB = 0 for J <= 1,
+// else J - 3/2), normalised at the resonance mass (or an effective mass for
+// non-resonant or above-limit masses).
+template <class T>
+PWA_HD T vertex_lambda(const Masses& P, int spin2, double mpp, T mr) {
+    const int LB = (spin2 <= 2) ? 0 : (spin2 - 3) / 2;
+    const double pB = breakup_d(P.m_lb, P.m_jpsi, mpp);
+    T pB0(pB);
+    const double mrv = val(mr);
+    const double mmin = P.m_k + P.m_p, mmax = P.m_lb - P.m_jpsi;
+    if (mrv < -0.0) pB0 = T(breakup_d(P.m_lb, P.m_jpsi, (mmin + mmax) / 2.0));
+    if (mrv > 1e-10 && mrv + P.m_jpsi < P.m_lb) pB0 = breakup_dm(P.m_lb, P.m_jpsi, mr);
+    if (mrv + P.m_jpsi > P.m_lb) pB0 = breakup_dm(P.m_lb, P.m_jpsi, tanh_mass(mr, mmin, mmax));
+    return barrier_ratio(P.ffb, LB, T(pB), pB0) * ipow(pB / P.m_lb, LB) * ipow(P.m_lb / pB0, LB);
+}
+
+// Lambda_b -> K- Pc vertex barrier term (LB = J - 1/2; no 1/pB0 normalisation).
+template <class T>
+PWA_HD T vertex_pc(const Masses& P, int spin2, double mpp, T mr) {
+    const int LB = (spin2 - 1) / 2;
+    const double pB = breakup_d(P.m_lb, P.m_k, mpp);
+    T pB0(pB);
+    const double mrv = val(mr);
+    if (mrv < -0.0) {
+        const double mmin = P.m_jpsi + P.m_p, mmax = P.m_lb - P.m_k;
+        pB0 = T(breakup_d(P.m_lb, P.m_k, (mmin + mmax) / 2.0));
+    }
+    if (mrv > 1e-10 && mrv + P.m_k < P.m_lb) pB0 = breakup_dm(P.m_lb, P.m_k, mr);
+    return barrier_ratio(P.ffb, LB, T(pB), pB0) * ipow(pB / P.m_lb, LB);
+}
+
+// Lambda_b -> p Zcs vertex barrier term (LB = (int)(J - 1/2)).
+template <class T>
+PWA_HD T vertex_zcs(const Masses& P, int spin2, double mpp, T mr) {
+    const int LB = spin2 >= 1 ? (spin2 - 1) / 2 : 0;
+    const double pB = breakup_d(P.m_lb, P.m_p, mpp);
+    const double mrv = val(mr);
+    if (mrv < -0.0) return T(ipow(pB / P.m_lb, LB));
+    T pB0(pB);
+    if (mrv > 1e-10 && mrv + P.m_p < P.m_lb) pB0 = breakup_dm(P.m_lb, P.m_p, mr);
+    return barrier_ratio(P.ffb, LB, T(pB), pB0) * ipow(pB / P.m_lb, LB);
+}
+
+} // namespace pwa

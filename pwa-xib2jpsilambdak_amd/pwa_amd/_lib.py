@@ -1,0 +1,202 @@
+"""ctypes binding of libpwa_amd.so (the C ABI of include/pwa_amp.h).
+
+The shared library is built in-tree (``pwa-xib2jpsilambdak_amd/build.sh``);
+loading fails loudly when it is missing -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpwa_amd.so")
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class PWAError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PWAError(f"{LIB_PATH} is not built; run pwa-xib2jpsilambdak_amd/build.sh "
+                           "or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        md = C.POINTER(abi.ModelDesc)
+        vp = C.c_void_p
+        sig = {
+            "pwa_fit_param_count": (C.c_int, [md, _ip]),
+            "pwa_create": (C.c_int, [md, C.c_int, C.POINTER(vp)]),
+            "pwa_destroy": (C.c_int, [vp]),
+            "pwa_last_error": (C.c_char_p, [vp]),
+            "pwa_upload_events": (C.c_int, [vp, C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ip]),
+            "pwa_eval": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
+            "pwa_eval_grad": (C.c_int, [vp, _dp, _dp, _dp]),
+            "pwa_set_constraints": (C.c_int, [vp, C.c_int32, _ip, _dp, _dp, _dp, _dp]),
+            "pwa_fcn": (C.c_int, [vp, _dp, _dp, _dp]),
+            "pwa_alpha": (C.c_int, [vp, _dp]),
+            "pwa_amp2": (C.c_int, [vp, C.c_int, _dp, _dp]),
+            "pwa_last_timing": (C.c_int, [vp, _dp, _dp]),
+            "pwa_generate_phsp": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, _dp]),
+            "pwa_generate_phsp_p4": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, _dp]),
+            "pwa_phsp_angles": (C.c_int, [C.c_int64, _dp, _dp]),
+            "pwa_helicity_couplings": (C.c_int, [md, _dp, _dp, _ip]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_ip)
+
+
+def check(rc, ctx=None, what=""):
+    if rc != abi.PWA_OK:
+        msg = lib().pwa_last_error(ctx).decode() if ctx else ""
+        raise PWAError(f"{what} failed with status {rc}: {msg}")
+
+
+def generate_phsp(n, seed, masses=None, m_mu=0.1056584):
+    """Uniform phase-space events, (PWA_NCOL, n) float64 columns."""
+    from .models import M_LB, M_JPSI, M_P, M_K
+    m_lb, m_jpsi, m_p, m_k = masses or (M_LB, M_JPSI, M_P, M_K)
+    cols = np.zeros((abi.PWA_NCOL, n), dtype=np.float64)
+    check(lib().pwa_generate_phsp(seed, n, m_lb, m_jpsi, m_p, m_k, m_mu, dptr(cols)), what="pwa_generate_phsp")
+    return cols
+
+
+def generate_phsp_p4(n, seed, masses=None, m_mu=0.1056584):
+    from .models import M_LB, M_JPSI, M_P, M_K
+    m_lb, m_jpsi, m_p, m_k = masses or (M_LB, M_JPSI, M_P, M_K)
+    p4 = np.zeros((n, 16), dtype=np.float64)
+    check(lib().pwa_generate_phsp_p4(seed, n, m_lb, m_jpsi, m_p, m_k, m_mu, dptr(p4)), what="pwa_generate_phsp_p4")
+    return p4
+
+
+def phsp_angles(p4):
+    p4 = np.ascontiguousarray(p4, dtype=np.float64)
+    n = p4.shape[0]
+    cols = np.zeros((abi.PWA_NCOL, n), dtype=np.float64)
+    check(lib().pwa_phsp_angles(n, dptr(p4), dptr(cols)), what="pwa_phsp_angles")
+    return cols
+
+
+def helicity_couplings(desc, params):
+    n = C.c_int32(0)
+    check(lib().pwa_helicity_couplings(C.byref(desc), None, None, C.byref(n)), what="pwa_helicity_couplings")
+    out = np.zeros(2 * n.value, dtype=np.float64)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    check(lib().pwa_helicity_couplings(C.byref(desc), dptr(p), dptr(out), C.byref(n)), what="pwa_helicity_couplings")
+    return out[0::2] + 1j * out[1::2]
+
+
+class Context:
+    """One device context: model, uploaded samples, evaluation entry points."""
+
+    def __init__(self, desc, device=0):
+        self.desc = desc
+        self._h = C.c_void_p()
+        check(lib().pwa_create(C.byref(desc), device, C.byref(self._h)), what="pwa_create")
+        self.n_params = desc.n_params
+        self.n = {abi.PWA_DATA: 0, abi.PWA_MC: 0}
+
+    def close(self):
+        if self._h:
+            lib().pwa_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, kind, cols, weight, scheme=None):
+        cols = np.ascontiguousarray(cols, dtype=np.float64)
+        if cols.shape[0] != abi.PWA_NCOL:
+            raise ValueError(f"cols must have {abi.PWA_NCOL} rows")
+        n = cols.shape[1]
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        if w.shape != (n,):
+            raise ValueError("weight must have one entry per event")
+        sc = None if scheme is None else np.ascontiguousarray(scheme, dtype=np.int32)
+        check(lib().pwa_upload_events(self._h, kind, n, dptr(cols), n, dptr(w), iptr(sc)), self._h,
+              "pwa_upload_events")
+        self.n[kind] = n
+
+    def eval(self, params, want_fx=False):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        nll = C.c_double()
+        integ = C.c_double()
+        fxd = np.zeros(self.n[abi.PWA_DATA]) if want_fx else None
+        fxm = np.zeros(self.n[abi.PWA_MC]) if want_fx else None
+        check(lib().pwa_eval(self._h, dptr(p), C.byref(nll), C.byref(integ), dptr(fxd), dptr(fxm)), self._h,
+              "pwa_eval")
+        if want_fx:
+            return nll.value, integ.value, fxd, fxm
+        return nll.value, integ.value
+
+    def eval_grad(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        nll = C.c_double()
+        g = np.zeros(self.n_params)
+        check(lib().pwa_eval_grad(self._h, dptr(p), C.byref(nll), dptr(g)), self._h, "pwa_eval_grad")
+        return nll.value, g
+
+    def fcn(self, params, grad=False):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        v = C.c_double()
+        g = np.zeros(self.n_params) if grad else None
+        check(lib().pwa_fcn(self._h, dptr(p), C.byref(v), dptr(g)), self._h, "pwa_fcn")
+        return (v.value, g) if grad else v.value
+
+    def alpha(self):
+        a = C.c_double()
+        check(lib().pwa_alpha(self._h, C.byref(a)), self._h, "pwa_alpha")
+        return a.value
+
+    def set_constraints(self, idx, mu1, mu2, s1, s2):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (mu1, mu2, s1, s2)]
+        check(lib().pwa_set_constraints(self._h, len(idx), iptr(idx), *[dptr(a) for a in arrs]), self._h,
+              "pwa_set_constraints")
+
+    def amp2(self, kind, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        out = np.zeros(self.n[kind])
+        check(lib().pwa_amp2(self._h, kind, dptr(p), dptr(out)), self._h, "pwa_amp2")
+        return out
+
+    def last_timing(self):
+        t = C.c_double()
+        m = C.c_double()
+        check(lib().pwa_last_timing(self._h, C.byref(t), C.byref(m)), self._h, "pwa_last_timing")
+        return t.value, m.value

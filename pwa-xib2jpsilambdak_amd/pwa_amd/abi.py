@@ -1,0 +1,91 @@
+"""ctypes mirror of include/pwa_amp.h (data layout and constants).
+
+The struct layout here must match ``pwa_model_desc`` byte for byte; the
+test ``tests/test_abi.py`` checks the size and field offsets against the
+compiled library.
+"""
+import ctypes as C
+
+PWA_OK = 0
+PWA_E_ARG, PWA_E_MODEL, PWA_E_HIP, PWA_E_STATE, PWA_E_NOMEM = -1, -2, -3, -4, -5
+PWA_CHAIN_LAMBDA, PWA_CHAIN_PC, PWA_CHAIN_ZCS = 0, 1, 2
+PWA_BREITWIGNER = 1
+PWA_KMATRIX = 240
+PWA_NONRESONANT = 361
+
+PWA_MAX_RES = 64
+PWA_MAX_SCHEMES = 8
+PWA_MAX_SUBBINS = 160
+PWA_KM_NCH = 4
+PWA_KM_NPOLE = 4
+PWA_KM_NCLASS = 4
+
+PWA_F_NEG_FX_PENALTY = 0x1
+PWA_F_NOINT_PENALTY = 0x2
+PWA_F_KM_CLAMP = 0x4
+PWA_F_KM_FULL_F = 0x8
+
+PWA_NCOL = 24
+PWA_DATA, PWA_MC = 0, 1
+PWA_NPAR_LAMBDA, PWA_NPAR_PC, PWA_NPAR_ZCS = 19, 17, 15
+
+COLUMNS = [
+    "mLk", "mjpsiL", "mjpsik",
+    "cosTheta_Lb", "cosTheta_L", "cosTheta_Jpsi", "phiK", "phiMu",
+    "Z_cosTheta_Lb", "Z_cosTheta_Z", "Z_cosTheta_Jpsi", "Z_phiZ", "Z_phiJpsi", "Z_phiMu",
+    "cosTheta_Lambda", "alpha_Mu",
+    "X_cosTheta_Lb", "X_cosTheta_X", "X_cosTheta_Jpsi", "X_phiX", "X_phiJpsi", "X_phiMu",
+    "X_cosTheta_p", "X_alpha_Mu",
+]
+COL = {name: i for i, name in enumerate(COLUMNS)}
+
+_d4x4 = (C.c_double * PWA_KM_NCH) * PWA_KM_NCLASS
+_i4x4 = (C.c_int32 * PWA_KM_NCH) * PWA_KM_NCLASS
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("n_lambda", C.c_int32), ("n_pc", C.c_int32), ("n_zcs", C.c_int32),
+        ("type", C.c_int32 * PWA_MAX_RES),
+        ("spin2", C.c_int32 * PWA_MAX_RES),
+        ("parity", C.c_int32 * PWA_MAX_RES),
+        ("m_lb", C.c_double), ("m_p", C.c_double), ("m_k", C.c_double), ("m_jpsi", C.c_double),
+        ("ffr", C.c_double), ("ffb", C.c_double),
+        ("km_ma", _d4x4), ("km_mb", _d4x4), ("km_hw", _d4x4), ("km_cplx", _i4x4),
+        ("km_shift_ma", C.c_double * PWA_KM_NCH),
+        ("km_shift_mb", C.c_double * PWA_KM_NCH),
+        ("km_shift_hw", C.c_double * PWA_KM_NCH),
+        ("n_schemes", C.c_int32),
+        ("scheme_nbins", C.c_int32 * PWA_MAX_SCHEMES),
+        ("scheme_z", (C.c_double * PWA_MAX_SUBBINS) * PWA_MAX_SCHEMES),
+        ("mres", C.c_double * 3),
+        ("flags", C.c_uint32),
+        ("nll_shift", C.c_double),
+    ]
+
+    @property
+    def n_res(self):
+        return self.n_lambda + self.n_pc + self.n_zcs
+
+    def chain(self, r):
+        if r < self.n_lambda:
+            return PWA_CHAIN_LAMBDA
+        if r < self.n_lambda + self.n_pc:
+            return PWA_CHAIN_PC
+        return PWA_CHAIN_ZCS
+
+    def param_offset(self, r):
+        nl, npc = self.n_lambda, self.n_pc
+        if r < nl:
+            return r * PWA_NPAR_LAMBDA
+        if r < nl + npc:
+            return nl * PWA_NPAR_LAMBDA + (r - nl) * PWA_NPAR_PC
+        return nl * PWA_NPAR_LAMBDA + npc * PWA_NPAR_PC + (r - nl - npc) * PWA_NPAR_ZCS
+
+    def param_size(self, r):
+        return (PWA_NPAR_LAMBDA, PWA_NPAR_PC, PWA_NPAR_ZCS)[self.chain(r)]
+
+    @property
+    def n_params(self):
+        return (self.n_lambda * PWA_NPAR_LAMBDA + self.n_pc * PWA_NPAR_PC
+                + self.n_zcs * PWA_NPAR_ZCS)

@@ -1,0 +1,342 @@
+"""Model descriptions and fit-parameter vectors for the Lambda_b -> J/psi p K- amplitude.
+
+A model is a ``ModelDesc`` (the runtime replacement of the reference's
+compile-time resonance counts and constant tables) plus a fit-parameter vector
+laid out as the concatenation of the reference's per-resonance RooArgLists
+(RooTDalitz/ResL_fullPHSP_finnerBinning.h:22-37 for Lambda*,
+RooTDalitz/ResZKM_alternateCoupling.h:18-43 for Pc, RooTDalitz/ResX.h:1-28 for
+Zcs).  ``float_mask`` marks the entries a fit varies.
+
+The seeded workloads built here:
+  * ``config_small``  -- 5 Lambda* Breit-Wigners, 20k data / 100k MC (CPU-runnable case)
+  * ``config_nominal``-- 14 Lambda*, 3 narrow Pc Breit-Wigners, 1 Zcs, 40-sub-bin smearing
+  * ``config_kmatrix``-- the nominal model with a 4-channel K-matrix Pc sector
+  * ``reference_preset`` -- the reference's own constants (masses, K-matrix
+    channels, smearing schemes) for quirk-for-quirk parity runs.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+from statistics import NormalDist
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Particle masses of the nominal workloads (GeV).
+M_LB, M_JPSI, M_P, M_K, M_MU = 5.6196, 3.0969, 0.938272, 0.493677, 0.1056584
+
+# The reference amplitude's masses (src/cu_DPFPropogator.cu:11-14).
+REF_M_LB, REF_M_K, REF_M_P, REF_M_JPSI = 5.61951, 0.493677, 0.938272046, 3.096916
+
+# PDG masses / half widths of the K-matrix channels of the nominal K-matrix model.
+SIGC_P, SIGC_P_HW = 2.45265, 0.0023 / 2   # Sigma_c(2455)+
+SIGC_PP, SIGC_PP_HW = 2.45397, 0.00189 / 2  # Sigma_c(2455)++
+D0, DST0, DM = 1.86484, 2.00685, 1.86966
+
+
+def reference_constants():
+    with open(os.path.join(_HERE, "data", "reference_constants.json")) as f:
+        return json.load(f)
+
+
+def gaussian_quantiles(nbins: int) -> list[float]:
+    """Sub-bin offsets z_b = Phi^-1((b + 1/2)/nbins) of an equal-weight Gaussian smearing."""
+    nd = NormalDist()
+    return [nd.inv_cdf((b + 0.5) / nbins) for b in range(nbins)]
+
+
+@dataclass
+class Resonance:
+    chain: int
+    type: int
+    spin2: int
+    parity: int
+    params: list[float]
+    floating: list[bool] = field(default_factory=list)
+
+
+def _set_km_reference_channels(d: abi.ModelDesc):
+    """The reference's averaged K-matrix channels (RooTDalitz/Resonances.h:62-71,
+    src/cu_DPFPropogator.cu:591-680): 0 J/psi p, then Sigma_c(*) D(*) channels."""
+    Dst, D, Sc, Scs = 2.008555, 1.86719, 2.453435, 2.517955
+    Sc_hw, Scs_hw = 0.00105, 0.00800
+    for ik in range(4):
+        d.km_ma[ik][0], d.km_mb[ik][0], d.km_hw[ik][0], d.km_cplx[ik][0] = 0.938272046, 3.096916, 0.0, 0
+    # 1/2-: (D* Sc), (D* Sc*), (D Sc) complex mass
+    chans = {0: [(Dst, Sc, Sc_hw), (Dst, Scs, Scs_hw), (D, Sc, Sc_hw)],
+             1: [(Dst, Sc, Sc_hw), (Dst, Scs, Scs_hw), (D, Scs, Scs_hw)]}
+    for ik, lst in chans.items():
+        for c, (ma, mb, hw) in enumerate(lst, start=1):
+            d.km_ma[ik][c], d.km_mb[ik][c], d.km_hw[ik][c], d.km_cplx[ik][c] = ma, mb, hw, 1
+    for ik in (2, 3):
+        for c, (ma, mb) in enumerate([(Dst, Sc), (D, Sc), (D, Scs)], start=1):
+            d.km_ma[ik][c], d.km_mb[ik][c], d.km_hw[ik][c], d.km_cplx[ik][c] = ma, mb, 0.0, 0
+    # threshold-shift channels by index (src/cu_DPFPropogator.cu:695-714)
+    for c, (ma, mb, hw) in enumerate([(Dst, Sc, Sc_hw), (D, Scs, Scs_hw), (D, Sc, Sc_hw)], start=1):
+        d.km_shift_ma[c], d.km_shift_mb[c], d.km_shift_hw[c] = ma, mb, hw
+
+
+def base_desc(masses=(M_LB, M_P, M_K, M_JPSI), ffr=3.0, ffb=3.0, flags=0, nll_shift=0.0) -> abi.ModelDesc:
+    d = abi.ModelDesc()
+    d.m_lb, d.m_p, d.m_k, d.m_jpsi = masses
+    d.ffr, d.ffb = ffr, ffb
+    d.flags = flags
+    d.nll_shift = nll_shift
+    _set_km_reference_channels(d)
+    return d
+
+
+def set_schemes(d: abi.ModelDesc, schemes: list[list[float]], mres=(0.0, 0.0, 0.0)):
+    d.n_schemes = len(schemes)
+    for s, z in enumerate(schemes):
+        if len(z) > abi.PWA_MAX_SUBBINS:
+            raise ValueError("too many sub-bins")
+        d.scheme_nbins[s] = len(z)
+        for b, v in enumerate(z):
+            d.scheme_z[s][b] = v
+    d.mres[0], d.mres[1], d.mres[2] = mres
+
+
+def reference_preset(flags=abi.PWA_F_NEG_FX_PENALTY | abi.PWA_F_NOINT_PENALTY | abi.PWA_F_KM_CLAMP,
+                     nll_shift=1e7) -> abi.ModelDesc:
+    """Descriptor with the reference's constants: amplitude masses, FFR=FFB=3,
+    the 30/150/20-sub-bin schemes (ids 1/2/3) and the m(J/psi p) resolution
+    polynomial (RooTDalitz/Resonances.h:31-58), NLL_SHIFT=1e7 (Makefile:13)."""
+    rc = reference_constants()
+    d = base_desc(masses=(REF_M_LB, REF_M_P, REF_M_K, REF_M_JPSI), flags=flags, nll_shift=nll_shift)
+    set_schemes(d, [rc["smear_binning_scheme"], rc["smear_fine_binning_scheme"],
+                    rc["smear_bad_binning_scheme"]], mres=tuple(rc["mres"]))
+    return d
+
+
+# ------------------------------------------------------------- LS counting
+def n_ls_lambda(spin2: int) -> int:
+    """LS couplings of a Lambda* (src/RooDalitzAmplitude.cxx:5539-5545): 2S runs
+    |J-2|..J+2 (step 2), 2L = 2S -+ 1."""
+    return 2 * len(range(abs(spin2 - 2), spin2 + 3, 2))
+
+
+def ls_pc_indices(spin2: int, parity: int) -> int:
+    """Number of parity-allowed (S, L) couplings of a Pc (PartWaveCoffZ,
+    src/RooDalitzAmplitude.cxx:5624-5632)."""
+    n = 0
+    for S in (1, 3):
+        for L in range(abs(spin2 - S), spin2 + S + 1, 2):
+            if (-1) ** (L // 2 + 1) * parity < 0:
+                continue
+            n += 1
+    return n
+
+
+def _rand_c(rng, lo=0.1, hi=2.0):
+    r = rng.uniform(lo, hi)
+    ph = rng.uniform(-math.pi, math.pi)
+    return r * math.cos(ph), r * math.sin(ph)
+
+
+def lambda_res(rng, spin2, parity, m0, width, type_=abi.PWA_BREITWIGNER, sc=(1.0, 0.0)):
+    p = [0.0] * abi.PWA_NPAR_LAMBDA
+    fl = [False] * abi.PWA_NPAR_LAMBDA
+    for ib in range(n_ls_lambda(spin2)):
+        p[2 * ib], p[2 * ib + 1] = _rand_c(rng)
+        fl[2 * ib] = fl[2 * ib + 1] = True
+    p[12], p[13] = sc
+    p[14], p[15] = m0, width
+    fl[14] = fl[15] = type_ == abi.PWA_BREITWIGNER
+    p[16], p[17], p[18] = type_, spin2, parity
+    return Resonance(abi.PWA_CHAIN_LAMBDA, type_, spin2, parity, p, fl)
+
+
+def pc_res(rng, spin2, parity, m0, width, type_=abi.PWA_BREITWIGNER, g=None, fsc=None):
+    p = [0.0] * abi.PWA_NPAR_PC
+    fl = [False] * abi.PWA_NPAR_PC
+    p[0], p[1] = _rand_c(rng, 0.1, 1.0)
+    fl[0] = fl[1] = True
+    for ib in range(1, 1 + ls_pc_indices(spin2, parity)):
+        p[2 * ib], p[2 * ib + 1] = _rand_c(rng)
+        fl[2 * ib] = fl[2 * ib + 1] = True
+    if g is not None:
+        for i, v in enumerate(g):
+            p[8 + i] = v
+            fl[8 + i] = True
+    if fsc is not None:
+        for i, v in enumerate(fsc):
+            p[8 + i] = v
+            fl[8 + i] = True
+    p[12], p[13] = m0, width
+    fl[12] = type_ == abi.PWA_BREITWIGNER or (type_ == abi.PWA_KMATRIX and m0 > 0)
+    fl[13] = type_ == abi.PWA_BREITWIGNER
+    p[14], p[15], p[16] = type_, spin2, parity
+    return Resonance(abi.PWA_CHAIN_PC, type_, spin2, parity, p, fl)
+
+
+def zcs_res(rng, spin2, parity, m0, width):
+    p = [0.0] * abi.PWA_NPAR_ZCS
+    fl = [False] * abi.PWA_NPAR_ZCS
+    for ib in range(4):
+        p[2 * ib], p[2 * ib + 1] = _rand_c(rng)
+        fl[2 * ib] = fl[2 * ib + 1] = True
+    p[8], p[9] = _rand_c(rng, 0.1, 1.0)
+    fl[8] = fl[9] = True
+    p[10], p[11] = m0, width
+    fl[10] = fl[11] = True
+    p[12], p[13], p[14] = abi.PWA_BREITWIGNER, spin2, parity
+    return Resonance(abi.PWA_CHAIN_ZCS, abi.PWA_BREITWIGNER, spin2, parity, p, fl)
+
+
+@dataclass
+class Model:
+    desc: abi.ModelDesc
+    params: np.ndarray
+    float_mask: np.ndarray
+    name: str = ""
+    resonances: list = field(default_factory=list)
+
+    @property
+    def n_float(self):
+        return int(self.float_mask.sum())
+
+
+def assemble(desc: abi.ModelDesc, res: list[Resonance], name="") -> Model:
+    lam = [r for r in res if r.chain == abi.PWA_CHAIN_LAMBDA]
+    pcs = [r for r in res if r.chain == abi.PWA_CHAIN_PC]
+    zcs = [r for r in res if r.chain == abi.PWA_CHAIN_ZCS]
+    ordered = lam + pcs + zcs
+    if len(ordered) > abi.PWA_MAX_RES:
+        raise ValueError("too many resonances")
+    desc.n_lambda, desc.n_pc, desc.n_zcs = len(lam), len(pcs), len(zcs)
+    for i, r in enumerate(ordered):
+        desc.type[i], desc.spin2[i], desc.parity[i] = r.type, r.spin2, r.parity
+    params = np.array([v for r in ordered for v in r.params], dtype=np.float64)
+    mask = np.array([f for r in ordered for f in r.floating], dtype=bool)
+    return Model(desc, params, mask, name, ordered)
+
+
+# ------------------------------------------------------------- workloads
+SMALL_JP = [(1, -1), (1, 1), (3, -1), (3, 1), (5, 1)]
+NOMINAL_LAMBDA_JP = [(1, -1), (1, 1), (3, -1), (3, 1), (5, -1), (5, 1), (7, -1), (7, 1),
+                     (1, -1), (1, 1), (3, -1), (3, 1), (5, 1), (7, -1)]
+NOMINAL_PC = [(4.31, (1, -1)), (4.44, (1, -1)), (4.46, (3, -1))]
+
+
+def config_small(seed=1) -> Model:
+    """5 Lambda* relativistic Breit-Wigners, J^P 1/2-,1/2+,3/2-,3/2+,5/2+,
+    m ~ U(1.40, 2.35) GeV, Gamma ~ U(0.02, 0.30) GeV, r = 3 GeV^-1,
+    |c| ~ U(0.1, 2) with uniform phase."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    res = [lambda_res(rng, s2, p, rng.uniform(1.40, 2.35), rng.uniform(0.02, 0.30)) for s2, p in SMALL_JP]
+    # fix the overall phase and scale: first coupling of the first resonance
+    m = assemble(d, res, "small: 5 Lambda* BW")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
+def _nominal_lambda(rng):
+    return [lambda_res(rng, s2, p, rng.uniform(1.40, 2.58), rng.uniform(0.02, 0.30))
+            for s2, p in NOMINAL_LAMBDA_JP]
+
+
+def _nominal_smearing(d, nbins=40, sigma=0.0027):
+    set_schemes(d, [gaussian_quantiles(nbins)], mres=(sigma, 0.0, 0.0))
+
+
+SMEAR_WINDOW = (4.2, 4.6)
+
+
+def config_nominal(seed=2, nbins=40) -> Model:
+    """14 Lambda* (J <= 7/2, m in 1.40-2.58 GeV), 3 narrow Pc Breit-Wigners
+    (4.31/4.44/4.46 GeV, widths 5-20 MeV, J^P 1/2-,1/2-,3/2-), 1 Zcs (J^P = 1+)
+    in J/psi K-, Pc lineshapes smeared over `nbins` Gaussian sub-bins with
+    sigma = 2.7 MeV for events with m(J/psi p) in SMEAR_WINDOW."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    _nominal_smearing(d, nbins)
+    res = _nominal_lambda(rng)
+    for m0, (s2, p) in NOMINAL_PC:
+        res.append(pc_res(rng, s2, p, m0, rng.uniform(0.005, 0.020)))
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    m = assemble(d, res, "nominal: 14 Lambda* + 3 Pc + 1 Zcs, 40-bin smearing")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
+def config_kmatrix(seed=3, nbins=40) -> Model:
+    """The nominal model with its Pc sector replaced by a 4-channel K-matrix in
+    J^P = 1/2-: channels J/psi p, Sigma_c+ D0, Sigma_c+ D*0, Sigma_c++ D-; two
+    bare poles in 4.2-4.6 GeV; a real symmetric background f ~ N(0, 0.5)
+    (all 10 entries, PWA_F_KM_FULL_F); production through the four
+    non-resonant members' couplings."""
+    rng = np.random.default_rng(seed)
+    d = base_desc(flags=abi.PWA_F_KM_CLAMP)
+    _nominal_smearing(d, nbins)
+    chans = [(M_P, M_JPSI, 0.0, 0), (D0, SIGC_P, SIGC_P_HW, 1), (DST0, SIGC_P, SIGC_P_HW, 1),
+             (DM, SIGC_PP, SIGC_PP_HW, 1)]
+    for c, (ma, mb, hw, cp) in enumerate(chans):
+        d.km_ma[0][c], d.km_mb[0][c], d.km_hw[0][c], d.km_cplx[0][c] = ma, mb, hw, cp
+    for c in range(1, 4):
+        d.km_shift_ma[c], d.km_shift_mb[c], d.km_shift_hw[c] = chans[c][0], chans[c][1], chans[c][2]
+    res = _nominal_lambda(rng)
+    poles = sorted(rng.uniform(4.2, 4.6, size=2))
+    for ip, m0 in enumerate(poles):
+        g = rng.normal(0.0, 0.5, size=4)
+        r = pc_res(rng, 1, -1, m0, float(ip + 1), type_=abi.PWA_KMATRIX, g=list(g))
+        res.append(r)
+    f = rng.normal(0.0, 0.5, size=(4, 4))
+    f = 0.5 * (f + f.T)
+    for i in range(4):
+        # member i carries f[0][i] in slot 8 and f[i][i..3] (i >= 1) in slots 9..
+        vals = [f[0][i]] + ([f[i][j] for j in range(i, 4)] if i >= 1 else [])
+        res.append(pc_res(rng, 1, -1, -1.0, 0.0, type_=abi.PWA_KMATRIX, fsc=vals))
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    d.flags |= abi.PWA_F_KM_FULL_F
+    m = assemble(d, res, "kmatrix: 14 Lambda* + 4-channel K-matrix Pc (2 poles) + 1 Zcs")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
+def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
+    """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
+    m = np.asarray(mjpsip)
+    return np.where((m > window[0]) & (m < window[1]), scheme_id, 0).astype(np.int32)
+
+
+def reference_schemes(mjpsip_data: np.ndarray, mjpsip_mc: np.ndarray):
+    """The reference's event classification, for data already ordered as
+    gen_toys/rearrange_mJpsip/reOrder.C:6-10 leaves it.  Counts follow
+    src/RooDalitzAmplitude.cxx:391-397; the -1 on the 30-bin count is the
+    reference's call at :1945; ranges and the MC windows follow
+    src/kernel_calEva.cu:874-893,920-961.  Ids: 1 = 30-bin, 2 = 150-bin,
+    3 = 20-bin, 0 = unsmeared."""
+    md = np.asarray(mjpsip_data, dtype=np.float32).astype(np.float64)
+    mm = np.asarray(mjpsip_mc, dtype=np.float32).astype(np.float64)
+    win = (md > 4.2) & (md < 4.6)
+    fine = win & ~((md < 4.43) | (md > 4.47))
+    n_smr1 = int((win & ~fine).sum())
+    n_fine = int(fine.sum())
+    nda_smr = n_smr1 - 1
+
+    def norm(m):
+        return np.where(((m > 4.4) & (m < 4.48)) | ((m > 4.27) & (m < 4.34)), 3, 0)
+
+    ds = norm(md).astype(np.int32)
+    idx = np.arange(len(md))
+    ds[idx < nda_smr + n_fine] = 2
+    ds[idx < nda_smr] = 1
+    return ds, norm(mm).astype(np.int32), n_smr1, n_fine
+
+
+def reference_order(mjpsip_data: np.ndarray) -> np.ndarray:
+    """Permutation that orders data as reOrder.C does: 30-bin window events,
+    then fine-window events, then the rest (stable within each group)."""
+    m = np.asarray(mjpsip_data, dtype=np.float32).astype(np.float64)
+    win = (m > 4.2) & (m < 4.6)
+    fine = win & ~((m < 4.43) | (m > 4.47))
+    g = np.where(win & ~fine, 0, np.where(fine, 1, 2))
+    return np.argsort(g, kind="stable")

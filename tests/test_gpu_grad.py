@@ -1,0 +1,106 @@
+"""Analytic gradient (pwa_eval_grad, the adjoint kernel) vs central finite
+differences -- of the HIP NLL at h = 1e-5 relative (all floating
+parameters) and of the CPU FP64 oracle NLL (a parameter subset)."""
+import numpy as np
+import pytest
+
+import oracle
+from pwa_amd import abi, models
+from pwa_amd._lib import Context, generate_phsp
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(model, nd, nm, seed):
+    dcols = generate_phsp(nd, seed)
+    mcols = generate_phsp(nm, seed + 100)
+    rng = np.random.default_rng(seed)
+    sw = rng.uniform(0.5, 1.5, size=nd)
+    wmc = rng.uniform(0.8, 1.2, size=nm)
+    if model.desc.n_schemes > 0:
+        ds = models.window_schemes(dcols[abi.COL["mjpsiL"]])
+        ms = models.window_schemes(mcols[abi.COL["mjpsiL"]])
+    else:
+        ds = np.zeros(nd, np.int32)
+        ms = np.zeros(nm, np.int32)
+    return dcols, sw, ds, mcols, wmc, ms
+
+
+def _h(x):
+    return 1e-5 * max(abs(x), 1e-2)
+
+
+def _fd(fun, p, idx):
+    g = np.zeros(len(idx))
+    for q, k in enumerate(idx):
+        h = _h(p[k])
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        g[q] = (fun(pp) - fun(pm)) / (2 * h)
+    return g
+
+
+def _check_fd(model, nd, nm, seed, n_oracle=8):
+    dcols, sw, ds, mcols, wmc, ms = _setup(model, nd, nm, seed)
+    p = model.params.copy()
+    idx = np.flatnonzero(model.float_mask)
+    print(f"[grad] {model.name} nd={nd} nm={nm} nfloat={len(idx)}", flush=True)
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        nll_g, g = c.eval_grad(p)
+        nll_e, _ = c.eval(p)
+        assert abs(nll_g - nll_e) <= 1e-12 * abs(nll_e)
+        gfd = _fd(lambda x: c.eval(x)[0], p, idx)
+    ga = g[idx]
+    scale = np.max(np.abs(gfd))
+    err = np.abs(ga - gfd)
+    print(f"[grad] max|g|={scale:.3e} max abs err={err.max():.3e} max rel err="
+          f"{np.max(err / np.maximum(np.abs(gfd), 1e-3 * scale)):.3e}", flush=True)
+    assert np.all(err <= 1e-6 * scale + 2e-5 * np.abs(gfd))
+    # non-floating entries that are not continuous parameters get exactly 0
+    for r in range(model.desc.n_res):
+        off, size = model.desc.param_offset(r), model.desc.param_size(r)
+        assert g[off + size - 3] == 0.0 and g[off + size - 2] == 0.0 and g[off + size - 1] == 0.0
+    # against the oracle's numerical gradient on a subset
+    rng = np.random.default_rng(seed)
+    sub = rng.choice(idx, size=min(n_oracle, len(idx)), replace=False)
+    gor = _fd(lambda x: oracle.nll(model.desc, x, dcols, sw, ds, mcols, wmc, ms)["nll"], p, sub)
+    erro = np.abs(g[sub] - gor)
+    print(f"[grad] oracle-FD subset max abs err={erro.max():.3e}", flush=True)
+    assert np.all(erro <= 1e-6 * scale + 2e-5 * np.abs(gor))
+
+
+def test_grad_small():
+    _check_fd(models.config_small(1), 1500, 3000, 41)
+
+
+def test_grad_nominal():
+    _check_fd(models.config_nominal(2), 600, 1200, 42)
+
+
+def test_grad_kmatrix():
+    _check_fd(models.config_kmatrix(3), 400, 800, 43)
+
+
+def test_fcn_gradient_with_constraints():
+    model = models.config_small(1)
+    dcols, sw, ds, mcols, wmc, ms = _setup(model, 800, 1600, 44)
+    p = model.params.copy()
+    m0_idx = [model.desc.param_offset(r) + 14 for r in range(model.desc.n_res)]
+    mu1 = p[m0_idx] + 0.01   # every mass below its window: penalties active
+    mu2 = mu1 + 0.02
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        c.set_constraints(m0_idx, mu1, mu2, np.full(5, 0.005), np.full(5, 0.007))
+        v, g = c.fcn(p, grad=True)
+        nll, _ = c.eval(p)
+        a = c.alpha()
+        assert abs(a - oracle.alpha(sw)) <= 1e-14 * a
+        ref = oracle.fcn(nll, model.desc.nll_shift, a, p, m0_idx, mu1, mu2, np.full(5, 0.005), np.full(5, 0.007))
+        assert abs(v - ref) <= 1e-12 * abs(ref)
+        idx = np.array(m0_idx[:3] + [0 + 2, 19 + 4])
+        gfd = _fd(lambda x: c.fcn(x), p, idx)
+    assert np.allclose(g[idx], gfd, rtol=2e-5, atol=1e-6 * np.max(np.abs(gfd)))

@@ -1,0 +1,95 @@
+"""HIP path vs the CPU FP64 oracle on identical seeded inputs (through the C ABI)."""
+import numpy as np
+import pytest
+
+import oracle
+from pwa_amd import abi, models
+from pwa_amd._lib import Context, generate_phsp
+
+pytestmark = pytest.mark.gpu
+
+# FP64 end to end: the NLL agrees to 1e-9 relative (target of the build), the
+# per-event |A|^2 to 1e-11 relative (observed ~1e-14; the bound leaves room
+# for the different Wigner-d algorithms: log-factorial sums vs closed form).
+NLL_RTOL = 1e-9
+FX_RTOL = 1e-11
+
+
+def _sample(model, nd, nm, seed):
+    dcols = generate_phsp(nd, seed)
+    mcols = generate_phsp(nm, seed + 100)
+    rng = np.random.default_rng(seed)
+    sw = rng.uniform(0.5, 1.5, size=nd)
+    wmc = rng.uniform(0.8, 1.2, size=nm)
+    if model.desc.n_schemes > 0:
+        ds = models.window_schemes(dcols[abi.COL["mjpsiL"]])
+        ms = models.window_schemes(mcols[abi.COL["mjpsiL"]])
+    else:
+        ds = np.zeros(nd, np.int32)
+        ms = np.zeros(nm, np.int32)
+    return dcols, sw, ds, mcols, wmc, ms
+
+
+def _check(model, nd, nm, seed):
+    dcols, sw, ds, mcols, wmc, ms = _sample(model, nd, nm, seed)
+    ref = oracle.nll(model.desc, model.params, dcols, sw, ds, mcols, wmc, ms)
+    print(f"[parity] {model.name} nd={nd} nm={nm} seed={seed}", flush=True)
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        nll, integ, fxd, fxm = c.eval(model.params, want_fx=True)
+    rel_fx = max(np.max(np.abs(fxd - ref["fx_data"]) / np.abs(ref["fx_data"])),
+                 np.max(np.abs(fxm - ref["fx_mc"]) / np.abs(ref["fx_mc"])))
+    rel_nll = abs(nll - ref["nll"]) / abs(ref["nll"])
+    print(f"[parity] rel_fx={rel_fx:.3e} rel_nll={rel_nll:.3e} rel_I={abs(integ - ref['integral']) / ref['integral']:.3e}",
+          flush=True)
+    assert rel_fx < FX_RTOL
+    assert abs(integ - ref["integral"]) <= FX_RTOL * ref["integral"]
+    assert rel_nll < NLL_RTOL
+
+
+def test_small_lambda_only():
+    _check(models.config_small(1), 3000, 6000, 11)
+
+
+def test_nominal_bw_pc_zcs_smeared():
+    _check(models.config_nominal(2), 1500, 3000, 12)
+
+
+def test_kmatrix_pc():
+    _check(models.config_kmatrix(3), 800, 1600, 13)
+
+
+def test_reference_preset_quirks():
+    """Reference constants, 30/150/20-sub-bin schemes from the reference's
+    event classification, NLL_SHIFT and both penalties on."""
+    base = models.config_nominal(4)
+    d = models.reference_preset()
+    for f in ("n_lambda", "n_pc", "n_zcs", "type", "spin2", "parity"):
+        setattr(d, f, getattr(base.desc, f))
+    m = models.Model(d, base.params, base.float_mask, "reference preset")
+    dcols = generate_phsp(1500, 21)
+    order = models.reference_order(dcols[abi.COL["mjpsiL"]])
+    dcols = dcols[:, order]
+    mcols = generate_phsp(3000, 22)
+    ds, ms, _, _ = models.reference_schemes(dcols[abi.COL["mjpsiL"]], mcols[abi.COL["mjpsiL"]])
+    rng = np.random.default_rng(5)
+    sw = rng.uniform(0.5, 1.5, size=dcols.shape[1])
+    wmc = np.ones(mcols.shape[1])
+    ref = oracle.nll(d, m.params, dcols, sw, ds, mcols, wmc, ms)
+    with Context(d) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        nll, integ, fxd, fxm = c.eval(m.params, want_fx=True)
+    assert np.max(np.abs(fxd - ref["fx_data"]) / np.abs(ref["fx_data"])) < FX_RTOL
+    assert abs(nll - ref["nll"]) / abs(ref["nll"]) < NLL_RTOL
+
+
+def test_unsmeared_amp2_matches_oracle():
+    model = models.config_nominal(2)
+    cols = generate_phsp(2000, 31)
+    ref = oracle.amp2(model.desc, model.params, cols)
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_MC, cols, np.ones(cols.shape[1]))
+        a2 = c.amp2(abi.PWA_MC, model.params)
+    assert np.max(np.abs(a2 - ref) / ref) < FX_RTOL

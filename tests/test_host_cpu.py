@@ -1,0 +1,177 @@
+"""CPU-side checks of the product library: exported C ABI, the LS -> helicity
+coupling map against the oracle's restatement, the phase-space generator and
+the helicity-angle calculator against an independent numpy restatement of
+gen_toys/phsp_toys/tree.C:127-408, and argument validation (no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from pwa_amd import abi, models
+from pwa_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "pwa_amp.h")).read()
+    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(pwa_[a-z0-9_]+)\s*\(", hdr, re.M))
+    assert len(names) >= 15
+    L = C.CDLL(_lib.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(L, n), n
+
+
+def test_model_struct_layout_matches_header():
+    # field offsets of the ctypes mirror must match the C struct; the library
+    # reads n_schemes / scheme_nbins / flags through the same layout
+    assert abi.ModelDesc.n_lambda.offset == 0
+    assert C.sizeof(abi.ModelDesc) % 8 == 0
+    # pwa_fit_param_count reads the counts through the C layout
+    m = models.config_nominal(2)
+    n = C.c_int32()
+    assert _lib.lib().pwa_fit_param_count(C.byref(m.desc), C.byref(n)) == 0
+    assert n.value == m.desc.n_params == len(m.params)
+
+
+@pytest.mark.parametrize("make", [models.config_small, models.config_nominal, models.config_kmatrix])
+def test_helicity_couplings_match_oracle(make):
+    m = make()
+    ref = oracle.build_amp(m.desc, m.params)["vamp"]
+    got = _lib.helicity_couplings(m.desc, m.params)
+    assert len(got) == len(ref)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+def test_helicity_couplings_reference_preset_all_spins():
+    rng = np.random.default_rng(5)
+    d = models.reference_preset()
+    res = [models.lambda_res(rng, j2, p, 1.6, 0.05) for j2 in (1, 3, 5, 7, 9) for p in (-1, 1)]
+    res += [models.pc_res(rng, j2, p, 4.4, 0.02) for j2 in (1, 3, 5, 7) for p in (-1, 1)]
+    res += [models.zcs_res(rng, j2, p, 4.0, 0.1) for j2 in (0, 2, 4) for p in (-1, 1)]
+    m = models.assemble(d, res)
+    ref = oracle.build_amp(m.desc, m.params)["vamp"]
+    got = _lib.helicity_couplings(m.desc, m.params)
+    assert len(got) == len(ref)
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+def test_invalid_models_rejected():
+    d = models.base_desc()
+    d.n_lambda = 1
+    d.type[0] = 360  # revised K-matrix: not supported -> error, no fallback
+    d.spin2[0] = 1
+    d.parity[0] = 1
+    n = C.c_int32()
+    assert _lib.lib().pwa_helicity_couplings(C.byref(d), None, None, C.byref(n)) == abi.PWA_E_MODEL
+    d.type[0] = abi.PWA_BREITWIGNER
+    d.parity[0] = 0
+    assert _lib.lib().pwa_helicity_couplings(C.byref(d), None, None, C.byref(n)) == abi.PWA_E_MODEL
+
+
+# ------------------------------------------------------- kinematics
+def _boost(v, b):
+    b2 = np.sum(b * b, axis=-1, keepdims=True)
+    g = 1 / np.sqrt(1 - b2)
+    bp = np.sum(b * v[..., :3], axis=-1, keepdims=True)
+    g2 = np.where(b2 > 0, (g - 1) / np.where(b2 > 0, b2, 1), 0)
+    p = v[..., :3] + (g2 * bp + g * v[..., 3:4]) * b
+    e = g * (v[..., 3:4] + bp)
+    return np.concatenate([p, e], axis=-1)
+
+
+def _unit(x):
+    return x / np.linalg.norm(x, axis=-1, keepdims=True)
+
+
+def _dot(a, b):
+    return np.sum(a * b, axis=-1)
+
+
+def _tree_angles(p4):
+    """Independent numpy restatement of tree.C helicityJpsiLam / helicityZK /
+    helicityTwoFrame for the Lambda* and Pc chains."""
+    mup, mum, pr, ka = p4[:, 0:4], p4[:, 4:8], p4[:, 8:12], p4[:, 12:16]
+    psi, lam = mup + mum, pr + ka
+    B = psi + lam
+    out = {}
+    PBu = _unit(B[:, :3])
+    bB = -B[:, :3] / B[:, 3:4]
+    mu2, p, J, L = _boost(mum, bB), _boost(pr, bB), _boost(psi, bB), _boost(lam, bB)
+    Lu, psiu = _unit(L[:, :3]), _unit(J[:, :3])
+    out["cosTheta_Lb"] = _dot(PBu, Lu)
+    p = _boost(p, -L[:, :3] / L[:, 3:4])
+    U = _unit(p[:, :3])
+    out["cosTheta_L"] = -_dot(psiu, U)
+    ab = -_unit(B[:, :3] - _dot(B[:, :3], Lu)[:, None] * Lu)
+    out["phiK"] = np.arctan2(_dot(_unit(np.cross(-psiu, ab)), U), _dot(ab, U))
+    ab = -ab
+    mu2 = _boost(mu2, -J[:, :3] / J[:, 3:4])
+    U2 = _unit(mu2[:, :3])
+    out["cosTheta_Jpsi"] = _dot(psiu, U2)
+    out["phiMu"] = np.arctan2(_dot(_unit(np.cross(psiu, ab)), U2), _dot(ab, U2))
+    # Pc chain
+    Z = _boost(psi + pr, bB)
+    Zu = _unit(Z[:, :3])
+    out["Z_cosTheta_Lb"] = _dot(PBu, Zu)
+    ab = _unit(L[:, :3] - _dot(L[:, :3], PBu)[:, None] * PBu)
+    out["Z_phiZ"] = np.arctan2(_dot(_unit(np.cross(PBu, ab)), Zu), _dot(ab, Zu))
+    bZ = -Z[:, :3] / Z[:, 3:4]
+    Jz, mz, kz = _boost(_boost(psi, bB), bZ), _boost(_boost(mum, bB), bZ), _boost(_boost(ka, bB), bZ)
+    U = _unit(Jz[:, :3])
+    out["Z_cosTheta_Z"] = -_dot(_unit(kz[:, :3]), U)
+    ab = -_unit(B[:, :3] - _dot(B[:, :3], Zu)[:, None] * Zu)
+    out["Z_phiJpsi"] = np.arctan2(_dot(_unit(np.cross(-_unit(kz[:, :3]), ab)), U), _dot(ab, U))
+    mz = _boost(mz, -Jz[:, :3] / Jz[:, 3:4])
+    U2 = _unit(mz[:, :3])
+    out["Z_cosTheta_Jpsi"] = _dot(U, U2)
+    ab = -_unit(-kz[:, :3] + _dot(kz[:, :3], U)[:, None] * U)
+    out["Z_phiMu"] = np.arctan2(_dot(_unit(np.cross(U, ab)), U2), _dot(ab, U2))
+    # alignment
+    pB, kB = _boost(pr, bB), _boost(ka, bB)
+    bp = -pB[:, :3] / pB[:, 3:4]
+    out["cosTheta_Lambda"] = _dot(_unit(_boost(kB, bp)[:, :3]), _unit(_boost(J, bp)[:, :3]))
+    bJ = -J[:, :3] / J[:, 3:4]
+    z3 = _unit(_boost(_boost(mum, bB), bJ)[:, :3])
+    pJ, LJ = _boost(pB, bJ)[:, :3], _boost(L, bJ)[:, :3]
+    x3Z = -_unit(-pJ + _dot(pJ, z3)[:, None] * z3)
+    x3L = -_unit(-LJ + _dot(LJ, z3)[:, None] * z3)
+    out["alpha_Mu"] = np.arctan2(_dot(_unit(np.cross(z3, x3Z)), x3L), _dot(x3Z, x3L))
+    return out
+
+
+def test_angles_match_tree_restatement():
+    p4 = _lib.generate_phsp_p4(4000, 3)
+    cols = _lib.phsp_angles(p4)
+    ref = _tree_angles(p4.reshape(-1, 16))
+    for name, v in ref.items():
+        got = cols[abi.COL[name]]
+        if name.startswith("phi") or "phi" in name or "alpha" in name:
+            dv = np.angle(np.exp(1j * (got - v)))
+        else:
+            dv = got - v
+        assert np.max(np.abs(dv)) < 1e-9, name
+
+
+def test_generator_kinematics():
+    p4 = _lib.generate_phsp_p4(20000, 4).reshape(-1, 4, 4)
+    masses = np.sqrt(np.maximum(p4[..., 3] ** 2 - np.sum(p4[..., :3] ** 2, axis=-1), 0))
+    assert np.allclose(masses[:, 2], models.M_P, atol=1e-9)
+    assert np.allclose(masses[:, 3], models.M_K, atol=1e-9)
+    tot = p4.sum(axis=1)
+    mlb = np.sqrt(tot[:, 3] ** 2 - np.sum(tot[:, :3] ** 2, axis=-1))
+    assert np.allclose(mlb, models.M_LB, atol=1e-8)
+    cols = _lib.phsp_angles(p4.reshape(-1, 16))
+    # flat Dalitz plane: m^2(pK) marginal of uniform phase space is the
+    # Dalitz-boundary length; compare two halves of the plane by area
+    s12 = cols[0] ** 2
+    lo, hi = (models.M_P + models.M_K) ** 2, (models.M_LB - models.M_JPSI) ** 2
+    frac = np.mean(s12 < 0.5 * (lo + hi))
+    assert 0.3 < frac < 0.7
+    for c in ("cosTheta_Lb", "cosTheta_L", "cosTheta_Jpsi", "Z_cosTheta_Z", "cosTheta_Lambda"):
+        assert np.all(np.abs(cols[abi.COL[c]]) <= 1 + 1e-12)
+    # cosTheta_Jpsi is uniform for isotropic J/psi decay
+    assert abs(np.mean(cols[abi.COL["cosTheta_Jpsi"]])) < 0.03

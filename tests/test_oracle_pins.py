@@ -1,0 +1,298 @@
+"""Pin the CPU FP64 oracle against independent mathematics and the
+reference-held constants (no GPU).  Each check is chosen so that a plausible
+slip (sign, index, transposed operand, dropped term) fails it."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+from scipy.special import spherical_jn, spherical_yn
+
+import oracle
+from pwa_amd import abi, models
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------- Wigner d
+def _jy(j):
+    """J_y in the |j m> basis, m = j, j-1, ..., -j (Condon-Shortley)."""
+    ms = np.arange(j, -j - 1, -1)
+    n = len(ms)
+    jp = np.zeros((n, n))
+    for a in range(1, n):  # <m+1|J+|m>
+        m = ms[a]
+        jp[a - 1, a] = math.sqrt(j * (j + 1) - m * (m + 1))
+    return (jp - jp.T) / 2j, ms
+
+
+def _d_expm(j, beta):
+    jy, ms = _jy(j)
+    return expm(-1j * beta * jy).real, ms
+
+
+@pytest.mark.parametrize("j2", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_dfun_matches_rotation_matrix(j2):
+    j = j2 / 2
+    rng = np.random.default_rng(j2)
+    for beta in list(rng.uniform(0.01, math.pi - 0.01, 5)) + [0.0, math.pi]:
+        d, ms = _d_expm(j, beta)
+        for a, m1 in enumerate(ms):
+            for b, m2 in enumerate(ms):
+                v = oracle.dfun(beta, j, m1, m2)
+                assert abs(v - d[a, b]) < 1e-12, (j, m1, m2, beta, v, d[a, b])
+
+
+def test_dfun_unitarity():
+    for j in (0.5, 1.5, 2.5, 3.5, 4.5):
+        ms = np.arange(j, -j - 1, -1)
+        for m1 in ms:
+            s = sum(oracle.dfun(1.234, j, m1, m2) ** 2 for m2 in ms)
+            assert abs(s - 1) < 1e-13
+
+
+def test_dfunhalf_dfunone_closed_forms():
+    L = oracle.lib()
+    for beta in (0.3, 1.7, 2.9):
+        d, ms = _d_expm(0.5, beta)
+        for a, m1 in enumerate(ms):
+            for b, m2 in enumerate(ms):
+                assert abs(L.orc_dfunhalf(beta, int(2 * m1), int(2 * m2)) - d[a, b]) < 1e-14
+        d, ms = _d_expm(1, beta)
+        for a, m1 in enumerate(ms):
+            for b, m2 in enumerate(ms):
+                if m2 == 0 and m1 != 0:
+                    continue  # the reference's dfunone covers n = +-1 and (0, 0) only
+                got = L.orc_dfunone(beta, int(m1), int(m2))
+                # the reference uses 7.07106781186547461715e-01 for 1/sqrt(2)
+                assert abs(got - d[a, b]) < 1e-15 + 2e-16 * abs(d[a, b])
+
+
+# ------------------------------------------------------ Clebsch-Gordan
+def test_cg_matches_sympy():
+    from sympy import Rational as R
+    from sympy.physics.quantum.cg import CG
+    for j1 in range(0, 7):
+        for j2 in range(0, 5):
+            for J in range(abs(j1 - j2), j1 + j2 + 1, 2):
+                for m1 in range(-j1, j1 + 1, 2):
+                    for m2 in range(-j2, j2 + 1, 2):
+                        M = m1 + m2
+                        if abs(M) > J:
+                            continue
+                        ref = float(CG(R(j1, 2), R(m1, 2), R(j2, 2), R(m2, 2), R(J, 2), R(M, 2)).doit())
+                        assert abs(oracle.cg(J, M, j1, j2, m1, m2) - ref) < 1e-13
+
+
+def test_c2_table_matches_reference_literals():
+    ref = json.load(open(os.path.join(GOLD, "reference_tables.json")))["C2"]
+    L = oracle.lib()
+    for iS in range(6):
+        for iL in range(2):
+            assert abs(L.orc_c2(iS, iL) - ref[iS][iL]) < 1e-13
+
+
+# -------------------------------------------------- barrier / lineshapes
+def _q_poly(L, z):
+    x = math.sqrt(z)
+    return x ** (2 * L + 2) * (spherical_jn(L, x) ** 2 + spherical_yn(L, x) ** 2)
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5])
+def test_blatt_weisskopf_from_spherical_hankel(L):
+    lib = oracle.lib()
+    r = 3.0
+    for p, p0 in [(0.2, 0.35), (0.9, 0.1), (1.3, 1.3)]:
+        got = lib.orc_fbw(r, L, p, p0)
+        ref = math.sqrt(_q_poly(L, (r * p0) ** 2) / _q_poly(L, (r * p) ** 2))
+        assert abs(got - ref) < 1e-11 * ref
+    assert lib.orc_fbw(r, L, 0.7, 0.7) == pytest.approx(1.0, abs=1e-15)
+
+
+def test_getp_kallen():
+    lib = oracle.lib()
+    M, m1, m2 = 5.6196, 3.0969, 1.9
+    lam = (M ** 2 - (m1 + m2) ** 2) * (M ** 2 - (m1 - m2) ** 2)
+    assert lib.orc_getp(M, m1, m2) == pytest.approx(math.sqrt(lam) / (2 * M), rel=1e-15)
+
+
+def test_breit_wigner_pole_value():
+    import ctypes as C
+    out = np.zeros(2)
+    oracle.lib().orc_breit_wigner(1.52, 0.016, 1.52 ** 2, out.ctypes.data_as(C.POINTER(C.c_double)))
+    assert out[0] == 0.0 and out[1] == pytest.approx(1 / (1.52 * 0.016), rel=1e-15)
+
+
+def test_bw_twobody_at_pole_is_normalisation_over_m_gamma():
+    import ctypes as C
+    d = models.base_desc()
+    out = np.zeros(2)
+    m, g = 4.44, 0.02
+    oracle.lib().orc_bw_amp(oracle.dref(d), 1, m, g, m, 1, out.ctypes.data_as(C.POINTER(C.c_double)))
+    kr = math.sqrt(m * m + g * g)
+    norm = 0.948850 * m * math.sqrt(g * kr / math.sqrt(m * (m + kr)))
+    assert out[0] == pytest.approx(0.0, abs=1e-12) and out[1] == pytest.approx(norm / (m * g), rel=1e-13)
+
+
+def test_complex_sqrt_branch_rule():
+    import ctypes as C
+    out = np.zeros(2)
+    p = out.ctypes.data_as(C.POINTER(C.c_double))
+    rng = np.random.default_rng(3)
+    for z in rng.normal(size=20) + 1j * rng.normal(size=20):
+        oracle.lib().orc_csqrt(z.real, z.imag, p)
+        ref = np.sqrt(z)
+        if z.imag < 0 and abs(z.imag / z.real) <= 1e-3:
+            ref = np.conj(ref)
+        assert abs(out[0] + 1j * out[1] - ref) < 1e-14 * abs(ref)
+    oracle.lib().orc_csqrt(4.0, -1e-4, p)    # |Im/Re| < 1e-3 with Im < 0 keeps +Im (reference rule)
+    assert out[1] > 0
+    oracle.lib().orc_csqrt(1e-7, -1e-7, p)   # inside the 1e-6 box -> 0
+    assert out[0] == 0 and out[1] == 0
+
+
+# --------------------------------------------------------------- K-matrix
+def test_solve_row0_matches_dense_inverse():
+    import ctypes as C
+    rng = np.random.default_rng(7)
+    for n in (2, 3, 4):
+        S = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 3 * np.eye(n)
+        flat = np.stack([S.real, S.imag], axis=-1).ravel()
+        out = np.zeros(2 * n)
+        oracle.lib().orc_solve_row0(flat.ctypes.data_as(C.POINTER(C.c_double)), n, out.ctypes.data_as(C.POINTER(C.c_double)))
+        got = out[0::2] + 1j * out[1::2]
+        assert np.allclose(got, np.linalg.inv(S)[0], rtol=1e-13, atol=1e-13)
+
+
+def _km_single_pole_model(g0, mpole, f00=0.0):
+    rng = np.random.default_rng(0)
+    d = models.base_desc()
+    res = [models.lambda_res(rng, 1, -1, 1.6, 0.1),
+           models.pc_res(rng, 1, -1, mpole, 0.0, type_=abi.PWA_KMATRIX, g=[g0, 0, 0, 0]),
+           models.pc_res(rng, 1, -1, -1.0, 0.0, type_=abi.PWA_KMATRIX, fsc=[f00])]
+    return models.assemble(d, res)
+
+
+def test_kmatrix_single_channel_single_pole_is_breit_wigner():
+    g0, mp = 0.6, 4.40
+    m = _km_single_pole_model(g0, mp)
+    for mpp in (4.2, 4.39, 4.45, 4.6):
+        Mtr = oracle.kmatrix(m.desc, m.params, mpp, 0)
+        s = mpp * mpp
+        ma, mb = m.desc.km_ma[0][0], m.desc.km_mb[0][0]
+        lam = (s - (ma + mb) ** 2) * (s - (ma - mb) ** 2)
+        rho0 = math.sqrt(abs(lam)) / s  # L = 0 for 1/2-
+        bw = g0 / (mp * mp - s - 1j * g0 * g0 * rho0)
+        assert abs(Mtr[0] - bw) < 1e-12 * abs(bw)
+        # non-resonant production through channel 0: (m^2 - s) T_0
+        assert abs(Mtr[1] - (mp * mp - s) / (mp * mp - s - 1j * g0 * g0 * rho0)) < 1e-12
+
+
+def test_kmatrix_unitarity_real_symmetric():
+    """Real symmetric K and real rho: S = 1 + 2i rho^1/2 T rho^1/2 is unitary,
+    with T = K (1 - i rho K)^-1; the oracle's row 0 must agree with it."""
+    rng = np.random.default_rng(11)
+    n = 4
+    K = rng.normal(size=(n, n))
+    K = K + K.T
+    rho = np.diag(rng.uniform(0.1, 0.9, n))
+    T = K @ np.linalg.inv(np.eye(n) - 1j * rho @ K)
+    Sm = np.eye(n) + 2j * np.sqrt(rho) @ T @ np.sqrt(rho)
+    assert np.allclose(Sm @ Sm.conj().T, np.eye(n), atol=1e-12)
+    import ctypes as C
+    A = np.eye(n) - 1j * K @ rho           # the oracle's (1 - i K rho), D = 1
+    flat = np.stack([A.real, A.imag], axis=-1).ravel()
+    out = np.zeros(2 * n)
+    oracle.lib().orc_solve_row0(flat.ctypes.data_as(C.POINTER(C.c_double)), n, out.ctypes.data_as(C.POINTER(C.c_double)))
+    row0 = out[0::2] + 1j * out[1::2]
+    # (1 - iK rho)^-1 K = K (1 - i rho K)^-1 = T  ->  row0 @ K = T[0]
+    assert np.allclose(row0 @ K, T[0], atol=1e-12)
+
+
+# -------------------------------------------------------- amplitude / NLL
+def _events(n, seed):
+    from pwa_amd._lib import generate_phsp
+    return generate_phsp(n, seed)
+
+
+def test_smearing_sigma_to_zero_equals_unsmeared():
+    m = models.config_nominal(2)
+    m.desc.mres[0] = 1e-13
+    cols = _events(200, 5)
+    a0 = oracle.amp2(m.desc, m.params, cols)
+    a1 = oracle.amp2(m.desc, m.params, cols, scheme=np.ones(200, np.int32))
+    assert np.allclose(a1, a0, rtol=1e-9)
+
+
+def _scale_all_couplings(model, kappa):
+    p = model.params.copy()
+    d = model.desc
+    for r in range(d.n_res):
+        o = d.param_offset(r)
+        ch = d.chain(r)
+        if ch == abi.PWA_CHAIN_LAMBDA:
+            sc = complex(p[o + 12], p[o + 13]) * kappa
+            p[o + 12], p[o + 13] = sc.real, sc.imag
+        else:
+            lo, hi = (1, 4) if ch == abi.PWA_CHAIN_PC else (0, 4)
+            for ib in range(lo, hi):
+                z = complex(p[o + 2 * ib], p[o + 2 * ib + 1]) * kappa
+                p[o + 2 * ib], p[o + 2 * ib + 1] = z.real, z.imag
+    return p
+
+
+def test_nll_invariant_under_global_coupling_rescaling():
+    m = models.config_nominal(2)
+    d, sw = _events(300, 8), np.random.default_rng(1).uniform(0.5, 1.5, 300)
+    mc, w = _events(600, 9), np.ones(600)
+    ds = models.window_schemes(d[1])
+    ms = models.window_schemes(mc[1])
+    r0 = oracle.nll(m.desc, m.params, d, sw, ds, mc, w, ms)
+    p1 = _scale_all_couplings(m, 1.7 * np.exp(0.6j))
+    r1 = oracle.nll(m.desc, p1, d, sw, ds, mc, w, ms)
+    assert np.allclose(r1["fx_data"], r0["fx_data"] * 1.7 ** 2, rtol=1e-12)
+    assert abs(r1["nll"] - r0["nll"]) < 1e-9 * abs(r0["nll"])
+    # MC weight scale
+    r2 = oracle.nll(m.desc, m.params, d, sw, ds, mc, 3.0 * w, ms)
+    assert abs(r2["nll"] - r0["nll"]) < 1e-9 * abs(r0["nll"])
+
+
+def test_nll_assembly_matches_fsum():
+    import mpmath
+    m = models.config_small(1)
+    d, sw = _events(400, 12), np.random.default_rng(2).uniform(0.5, 1.5, 400)
+    mc, w = _events(800, 13), np.random.default_rng(3).uniform(0.9, 1.1, 800)
+    z = np.zeros(400, np.int32), np.zeros(800, np.int32)
+    r = oracle.nll(m.desc, m.params, d, sw, z[0], mc, w, z[1])
+    I = mpmath.fsum(r["fx_mc"])
+    ref = float(mpmath.fsum(sw) * mpmath.log(I / mpmath.fsum(w)) - mpmath.fsum(
+        [mpmath.mpf(a) * mpmath.log(b) for a, b in zip(sw, r["fx_data"])]))
+    assert abs(r["nll"] - ref) < 1e-11 * abs(ref)
+    assert abs(r["integral"] - float(I)) < 1e-13 * float(I)
+
+
+def test_noint_term_is_incoherent_sum():
+    """With one resonance per chain the no-interference sum equals |A|^2 of
+    each chain alone; with a single Lambda* it equals the full |A|^2."""
+    rng = np.random.default_rng(4)
+    d = models.base_desc()
+    m = models.assemble(d, [models.lambda_res(rng, 3, -1, 1.52, 0.016)])
+    cols = _events(100, 14)
+    a = oracle.amp2(m.desc, m.params, cols)
+    b = oracle.amp2(m.desc, m.params, cols, interference=False)
+    assert np.allclose(a, b, rtol=1e-13)
+
+
+def test_reference_classification_helper_matches_oracle():
+    import ctypes as C
+    md = np.sort(_events(3000, 15)[1])[::-1].copy()
+    order = models.reference_order(md)
+    md = md[order]
+    mm = _events(2000, 16)[1]
+    ds, ms, n1, nf = models.reference_schemes(md, mm)
+    od, om = np.zeros(len(md), np.int32), np.zeros(len(mm), np.int32)
+    oracle.lib().orc_reference_schemes(len(md), oracle.dp(md), n1, nf, len(mm), oracle.dp(mm), oracle.ip(od), oracle.ip(om))
+    assert np.array_equal(ds, od) and np.array_equal(ms, om)
+    assert (ds == 1).sum() == n1 - 1 and (ds == 2).sum() >= nf
