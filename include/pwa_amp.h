@@ -228,9 +228,15 @@ int pwa_alpha(pwa_ctx* ctx, double* alpha);
    RooTDalitz/kernel_calEva.h:24, src/kernel_calEva.cu:1365-1419). */
 int pwa_amp2(pwa_ctx* ctx, int kind, const double* params, double* amp2);
 
-/* Device time of the last pwa_eval / pwa_eval_grad kernels, in ms (HIP events
-   on the context's stream), and the name of the dominant kernel.          */
+/* Device time of the last pwa_eval / pwa_eval_grad, in ms (HIP events on the
+   context's stream): all kernels, and the longest single kernel launch.   */
 int pwa_last_timing(pwa_ctx* ctx, double* ms_total, double* ms_main);
+
+/* Per-launch device times of the last evaluation: for launch q, ms[q], the
+   sample kind, smearing scheme, whether it was the gradient kernel, and its
+   event count.  *n receives the number of launches (up to cap are written). */
+int pwa_last_launches(pwa_ctx* ctx, int32_t cap, int32_t* n, double* ms, int32_t* kind,
+                      int32_t* scheme, int32_t* is_grad, int64_t* nevents);
 
 /*
  * Synthetic events: uniform Lambda_b -> J/psi p K- phase space with

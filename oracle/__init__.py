@@ -52,6 +52,9 @@ def lib():
             "orc_reference_schemes": (None, [C.c_int64, _dp, C.c_int64, C.c_int64, C.c_int64, _dp, _ip, _ip]),
             "orc_fcn": (C.c_double, [C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
             "orc_alpha": (C.c_double, [C.c_int64, _dp]),
+            "orc_prepare": (C.c_void_p, [C.c_int64, _dp, C.c_int64, _dp, _ip, C.c_int64, _dp, C.c_int64, _dp, _ip]),
+            "orc_free": (None, [C.c_void_p]),
+            "orc_nll_prepared": (C.c_int, [C.c_void_p, vp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -149,3 +152,26 @@ def fcn(nll_value, nll_shift, alpha_value, params, idx, mu1, mu2, s1, s2):
     idx = _i(idx)
     return lib().orc_fcn(nll_value, nll_shift, alpha_value, dp(_d(params)), len(idx), ip(idx), dp(_d(mu1)),
                          dp(_d(mu2)), dp(_d(s1)), dp(_d(s2)))
+
+
+class Prepared:
+    """Oracle NLL on tables filled once (the CPU-baseline form)."""
+
+    def __init__(self, dcols, sw, dscheme, mcols, wmc, mscheme):
+        self._keep = [_d(dcols), _d(sw), _i(dscheme), _d(mcols), _d(wmc), _i(mscheme)]
+        dc, s, ds, mc, w, ms = self._keep
+        self.h = lib().orc_prepare(dc.shape[1], dp(dc), dc.shape[1], dp(s), ip(ds), mc.shape[1], dp(mc), mc.shape[1],
+                                   dp(w), ip(ms))
+
+    def nll(self, desc, params):
+        out = np.zeros(1)
+        p = _d(params)
+        if lib().orc_nll_prepared(self.h, dref(desc), dp(p), dp(out)):
+            raise RuntimeError("orc_nll_prepared failed")
+        return float(out[0])
+
+    def __del__(self):
+        try:
+            lib().orc_free(self.h)
+        except Exception:
+            pass

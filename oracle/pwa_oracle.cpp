@@ -1052,13 +1052,25 @@ Phys phys_from(const pwa_model_desc* d) {
     return Phys{d->m_lb, d->m_k, d->m_p, d->m_jpsi, d->ffr, d->ffb};
 }
 
-double event_amp2(const Model& M, const AmpParams& ap, const Phys& P, const double* ev, int scheme, bool interference) {
-    Dlz z;
-    filldlz(z, ev);
+double table_amp2(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, int scheme, bool interference) {
     const bool clamp = (M.d->flags & PWA_F_KM_CLAMP) != 0;
     if (scheme > 0 && interference) return GetAmp2sSmear(M, ap, P, z, scheme, clamp);
     return GetAmp2s(M, ap, P, z, interference, clamp);
 }
+
+double event_amp2(const Model& M, const AmpParams& ap, const Phys& P, const double* ev, int scheme, bool interference) {
+    Dlz z;
+    filldlz(z, ev);
+    return table_amp2(M, ap, P, z, scheme, interference);
+}
+
+// Tables filled once per sample, as the reference does at load time
+// (src/RooDalitzAmplitude.cxx:319-502): the CPU-baseline form of the oracle.
+struct Prepared {
+    std::vector<Dlz> dz, mz;
+    std::vector<double> sw, w;
+    std::vector<int> ds, ms;
+};
 
 void gather(const double* cols, int64_t ld, int64_t i, double* ev) {
     for (int c = 0; c < PWA_NCOL; ++c) ev[c] = cols[c * ld + i];
@@ -1247,6 +1259,57 @@ int orc_nll(const pwa_model_desc* d, const double* par, int64_t nd, const double
     if (nll) *nll = v;
     if (integral) *integral = sumint;
     if (integral_noint) *integral_noint = sumnoint;
+    return 0;
+}
+
+void* orc_prepare(int64_t nd, const double* dcols, int64_t dld, const double* sw, const int32_t* dscheme, int64_t nm,
+                  const double* mcols, int64_t mld, const double* wmc, const int32_t* mscheme) {
+    Prepared* h = new Prepared();
+    double ev[PWA_NCOL];
+    h->dz.resize(nd);
+    h->mz.resize(nm);
+    for (int64_t i = 0; i < nd; ++i) {
+        gather(dcols, dld, i, ev);
+        filldlz(h->dz[i], ev);
+        h->sw.push_back(sw[i]);
+        h->ds.push_back(dscheme ? dscheme[i] : 0);
+    }
+    for (int64_t i = 0; i < nm; ++i) {
+        gather(mcols, mld, i, ev);
+        filldlz(h->mz[i], ev);
+        h->w.push_back(wmc[i]);
+        h->ms.push_back(mscheme ? mscheme[i] : 0);
+    }
+    return h;
+}
+
+void orc_free(void* h) { delete static_cast<Prepared*>(h); }
+
+// The NLL of orc_nll on prepared tables.
+int orc_nll_prepared(void* hp, const pwa_model_desc* d, const double* par, double* nll) {
+    Model M;
+    if (!hp || !model_from(d, M)) return -1;
+    const Prepared& h = *static_cast<Prepared*>(hp);
+    AmpParams ap;
+    build_amp(M, par, ap);
+    Phys P = phys_from(d);
+    double sumint = 0, sumnoint = 0, sumW = 0, sumW_data = 0, nll_data = 0;
+    for (size_t i = 0; i < h.mz.size(); ++i) {
+        const double a2 = table_amp2(M, ap, P, h.mz[i], h.ms[i], true);
+        const double w = h.w[i];
+        sumint += (w <= -1000.) ? a2 : a2 * w;
+        sumW += w;
+        if ((d->flags & PWA_F_NOINT_PENALTY) && !(w <= -1000.)) sumnoint += table_amp2(M, ap, P, h.mz[i], 0, false) * w;
+    }
+    const bool negpen = (d->flags & PWA_F_NEG_FX_PENALTY) != 0;
+    for (size_t i = 0; i < h.dz.size(); ++i) {
+        const double fx = table_amp2(M, ap, P, h.dz[i], h.ds[i], true);
+        sumW_data += h.sw[i];
+        nll_data += (fx > 0) ? std::log(fx) * h.sw[i] : (negpen ? -1e12 : 0.0);
+    }
+    double v = d->nll_shift + sumW_data * std::log(sumint / sumW) - nll_data;
+    if ((d->flags & PWA_F_NOINT_PENALTY) && std::fabs(sumnoint / sumint - 1.0) > 0.3) v += 1000.0;
+    *nll = v;
     return 0;
 }
 

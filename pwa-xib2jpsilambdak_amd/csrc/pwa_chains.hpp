@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "pwa_device.hpp"
+#include "pwa_kmatrix.hpp"
 
 namespace pwa {
 
@@ -194,9 +195,13 @@ __device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, cons
     C64 V[2][3][2];
     const C64* c = cpl + S->ampcount[r];
     int n = 0;
+#pragma unroll
     for (int k = 0; k < 2; ++k)
+#pragma unroll
         for (int ps = 0; ps < 3; ++ps)
+#pragma unroll
             for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
                 for (int ip = 0; ip < 2; ++ip) {
                     const int n2 = 2 * (ps - 1) - (2 * ip - 1);  // 2(helpsi - help)
                     if (n2 > J2 || n2 < -J2) continue;
@@ -231,10 +236,14 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
         C64 V[2][3][2];
         const C64* c = cpl + S->ampcount[r];
         int n = 0;
+#pragma unroll
         for (int k = 0; k < 2; ++k)
+#pragma unroll
             for (int ps = 0; ps < 3; ++ps) {
                 if (2 * abs(ps - 1) > J2) continue;
+#pragma unroll
                 for (int ip = 0; ip < 2; ++ip)
+#pragma unroll
                     for (int iz = 0; iz < 2; ++iz) {
                         const int hz2 = (2 * ip - 1) + (2 * iz - 1);  // 2 helZ = 2 help -+ 1
                         if (hz2 > J2 || hz2 < -J2) continue;
@@ -253,143 +262,6 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
     }
 }
 
-// ----------------------------------------------------------- K-matrix
-__device__ __forceinline__ C64 rho_real(double s, double m1, double m2, int L, double ffb) {
-    const double rho2 = (s - (m1 + m2) * (m1 + m2)) * (s - (m1 - m2) * (m1 - m2));
-    const double rho = sqrt(fabs(rho2)) / s * fbw2(s, m1, m2, L, ffb);
-    return rho2 >= 0 ? C64(rho, 0.0) : C64(0.0, rho);
-}
-__device__ __forceinline__ C64 rho_cmass(double s, double m1, double m2, double hw) {
-    const C64 a(m1 + m2, -hw), b(m1 - m2, hw);
-    const C64 p1 = a * a - C64(s, 0.0), p2 = b * b - C64(s, 0.0);
-    const C64 r = csqrt_branch(p1 * p2);
-    return C64(r.re / s, r.im / s);
-}
-
-// Production amplitudes of K-matrix class ik at mass mpp: Mtr[p] (poles) and
-// Mtr[np + c] (non-resonant channels), as Default_KMatrix
-// (src/cu_DPFPropogator.cu:588-813): T = row 0 of (D (1 - i K rho))^-1 with
-// K = sum_p g_p g_p^T / (m_p^2 - s) + f and D = prod_p (m_p^2 - s).
-static __device__ void kmatrix(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int ik, double mpp,
-                        C64 Mtr[8]) {
-    const int Lna = (ik >= 2) ? 1 : 0;
-    const double s = mpp * mpp;
-    C64 rho[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-        rho[c] = S->km_cplx[ik][c] ? rho_cmass(s, S->km_ma[ik][c], S->km_mb[ik][c], S->km_hw[ik][c])
-                                   : rho_real(s, S->km_ma[ik][c], S->km_mb[ik][c], Lna, S->P.ffb);
-    const double* g = D->km[ik];
-    const double* f = D->km[ik] + 16;
-    const int np = D->km_npole[ik];
-    double den[4] = {1.0, 1.0, 1.0, 1.0};
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        if (p >= np) continue;
-        const int r = D->km_pole_res[ik][p];
-        const double mfit = D->m0[r];
-        const int ch = D->km_pole_shift[ik][p];
-        double m02 = mfit * mfit;
-        if (ch > 0) {
-            const C64 rc = rho_cmass(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch]);
-            const double gc = g[p * 4 + ch];
-            const double m0 = sqrt(m02 - gc * gc * rc.im);
-            m02 = m0 * m0;
-        }
-        double d = m02 - s;
-        if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) d = 1e-3;
-        den[p] = d;
-    }
-    double Dall = 1.0, ms[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) Dall *= (p < np) ? den[p] : 1.0;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        double t = 1.0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (q != p && q < np) t *= den[q];
-        ms[p] = t;
-    }
-    double kma[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = a; b < 4; ++b) {
-            double v = f[a * 4 + b] * Dall;
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-                if (p < np) v += g[p * 4 + a] * (g[p * 4 + b] * ms[p]);
-            kma[a][b] = v;
-            kma[b][a] = v;
-        }
-    // A = KK^T, KK[a][b] = D delta_ab - i kma[a][b] rho[b]
-    C64 A[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            // A[a][b] = KK[b][a] = -i kma[b][a] rho[a] + D delta
-            const C64 t = rho[a] * kma[b][a];
-            A[a][b] = C64(t.im, -t.re);
-            if (a == b) A[a][b].re += Dall;
-        }
-    // Doolittle LU without pivoting, then solve A x = e0
-    C64 Lm[4][4], U[4][4], iU[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-#pragma unroll
-        for (int col = a; col < 4; ++col) {
-            C64 u = A[a][col];
-#pragma unroll
-            for (int k = 0; k < a; ++k) u = u - Lm[a][k] * U[k][col];
-            U[a][col] = u;
-        }
-        iU[a] = cinv(U[a][a]);
-#pragma unroll
-        for (int row = a + 1; row < 4; ++row) {
-            C64 l = A[row][a];
-#pragma unroll
-            for (int k = 0; k < a; ++k) l = l - Lm[row][k] * U[k][a];
-            Lm[row][a] = l * iU[a];
-        }
-    }
-    C64 y[4], T[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        C64 v = (a == 0) ? C64(1.0, 0.0) : C64(0.0, 0.0);
-#pragma unroll
-        for (int b = 0; b < a; ++b) v = v - Lm[a][b] * y[b];
-        y[a] = v;
-    }
-#pragma unroll
-    for (int a = 3; a >= 0; --a) {
-        C64 v = y[a];
-#pragma unroll
-        for (int b = a + 1; b < 4; ++b) v = v - U[a][b] * T[b];
-        T[a] = iU[a] * v;
-    }
-    const double na = Lna ? fbw2(s, S->km_ma[ik][0], S->km_mb[ik][0], Lna, S->P.ffb) : 1.0;
-#pragma unroll
-    for (int p = 0; p < 8; ++p) Mtr[p] = C64(0.0, 0.0);
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        if (p >= np) continue;
-        C64 acc;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * ms[p], T[c]);
-        Mtr[p] = acc * na;
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const C64 v = T[c] * (Dall * na);
-        // Mtr[np + c] = v, written with compile-time indices only
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == np + c) Mtr[q] = v;
-    }
-}
-
 __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
     C64 r = v[0];
 #pragma unroll
@@ -399,12 +271,13 @@ __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
 }
 
 // Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.
-template <int NZ>
+template <int NZ, bool KM = true>
 __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                               double m, C64 L[NZ]) {
     const int nR = S->nR;
 #pragma unroll
     for (int r = 0; r < NZ; ++r) L[r] = C64(1.0, 0.0);
+    if constexpr (KM)
     for (int c = 0; c < S->km_nclass; ++c) {
         const int ik = S->km_class[c];
         C64 Mtr[8];

@@ -85,17 +85,32 @@ struct Partial {
     double s1, c1;  // data: count of f <= 0     MC: sum w*f_noint
 };
 
-// Launch helpers (pwa_kernels.hip).
+// Arguments of the per-segment stage pipeline (pwa_kernels.hip).
+struct StageArgs {
+    const double* ev;
+    int64_t ld, n;
+    int scheme;
+    bool mc, grad;
+    bool km;              // the model has K-matrix members
+    int noint;
+    const DevStatic* S;
+    const DevDyn* D;
+    const C64* cpl;
+    double* inter;        // inter_planes(nz) x ld intermediate planes
+    double* fx;           // per-event output or null
+    Partial* part;        // shape_blocks partial sums
+    int shape_blocks, adj_blocks;
+    int ncpl, nres, ng;   // gradient layout
+    double* gpart_shape;  // shape_blocks x ng (gradient)
+    double* gpart_adj;    // adj_blocks x ng (gradient)
+};
+
 hipError_t launch_prep(const double* raw, int64_t ld_raw, int64_t n, double* ev, int64_t ld, hipStream_t st);
-hipError_t launch_nll(int nZ, bool mc, int scheme, const double* ev, int64_t ld, int64_t n, const DevStatic* S,
-                      const DevDyn* D, const C64* cpl, double* fx, Partial* part, int noint, hipStream_t st);
-hipError_t launch_reduce(const Partial* part, int nseg, const int32_t* seg_begin, const int32_t* seg_kind,
-                         double* out, hipStream_t st);
-hipError_t launch_grad(int nZ, bool mc, int scheme, const double* ev, int64_t ld, int64_t n, const DevStatic* S,
-                       const DevDyn* D, const C64* cpl, int ncpl, int nres, int NG, int blocks, double* gpart,
-                       Partial* part, int noint, hipStream_t st);
+hipError_t run_stages(int nz, const StageArgs& a, hipStream_t st);
+hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, double* out, hipStream_t st);
 hipError_t launch_grad_reduce(const double* gpart, const int32_t* beg, int NG, double* out, hipStream_t st);
 size_t grad_lds_bytes(int NG);
+int inter_planes(int nz);
 constexpr int kBlock = 256;
 constexpr int kGradNClassParams = 36;  // per K-matrix class: g[16], f[16], pole masses[4]
 

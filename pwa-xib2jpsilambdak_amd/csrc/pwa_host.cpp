@@ -25,6 +25,7 @@ struct Sample {
     int64_t n = 0, ld = 0;
     double* ev = nullptr;               // E_NPLANE x ld, device order
     double* fx = nullptr;               // per-event output scratch (device order)
+    double* inter = nullptr;            // per-event intermediate planes of the stage pipeline
     std::vector<int64_t> perm;          // device position -> caller index
     std::vector<int> seg_scheme;
     std::vector<int64_t> seg_begin, seg_count;
@@ -59,6 +60,13 @@ struct pwa_ctx {
     C64* hcpl = nullptr;      // pinned
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float last_ms = 0.f;
+    // per-launch timing of the last evaluation
+    static constexpr int kMaxLaunch = 32;
+    hipEvent_t le[kMaxLaunch][2] = {};
+    int nlaunch = 0;
+    int32_t l_kind[kMaxLaunch] = {}, l_scheme[kMaxLaunch] = {}, l_grad[kMaxLaunch] = {};
+    int64_t l_n[kMaxLaunch] = {};
+    float l_ms[kMaxLaunch] = {};
     std::string err;
     std::vector<int> cidx;
     std::vector<double> cmu1, cmu2, cs1, cs2;
@@ -228,21 +236,6 @@ void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
     }
 }
 
-int ensure_partials(pwa_ctx* c) {
-    int64_t need = 0;
-    for (int k = 0; k < 2; ++k)
-        for (int64_t cnt : c->smp[k].seg_count) need += (cnt + kBlock - 1) / kBlock;
-    need = std::max<int64_t>(need, 1);
-    if (need <= c->part_cap) return PWA_OK;
-    if (c->part) (void)hipFree(c->part);
-    c->part = nullptr;
-    HIPCHK(c, hipMalloc(&c->part, need * sizeof(Partial)));
-    c->part_cap = need;
-    return PWA_OK;
-}
-
-// Launch all segment kernels and the final reduction; returns the four
-// compensated sums per kind in c->hout (data: s0,c0,s1,c1; MC: s0,c0,s1,c1).
 int upload_params(pwa_ctx* c, const double* params) {
     // the previous evaluation has synchronised the stream, so the pinned
     // staging buffers are free to overwrite
@@ -256,40 +249,115 @@ int upload_params(pwa_ctx* c, const double* params) {
     return PWA_OK;
 }
 
-int run_eval(pwa_ctx* c, const double* params, bool want_fx, bool force_unsmeared, bool mc_as_data) {
-    if (!c->smp[PWA_DATA].present && !c->smp[PWA_MC].present) return fail(c, PWA_E_STATE, "no events uploaded");
+void collect_times(pwa_ctx* c) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
+    for (int q = 0; q < c->nlaunch; ++q) {
+        float t = 0.f;
+        c->l_ms[q] = (hipEventElapsedTime(&t, c->le[q][0], c->le[q][1]) == hipSuccess) ? t : -1.f;
+    }
+}
+
+constexpr int kMaxShapeBlocks = 2048;
+constexpr int kMaxAdjBlocks = 1024;
+
+// Runs the stage pipeline (k_amp, k_shape[, k_adj]) on every segment of both
+// samples, then the reductions; leaves the sums in c->hout (data: s0,c0,s1,c1;
+// MC: s0,c0,s1,c1) and, for the gradient, G_data and G_mc in c->hgout.
+int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool force_unsmeared, bool mc_as_data) {
     int rc = upload_params(c, params);
     if (rc) return rc;
-    rc = ensure_partials(c);
-    if (rc) return rc;
-    const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
-    int32_t beg[3] = {0, 0, 0};
-    int64_t pb = 0;
-    HIPCHK(c, hipEventRecord(c->e0, c->st));
+    const int NG = c->NG;
+    struct Seg { int k; size_t q; int sb, ab; };
+    std::vector<Seg> segs;
+    int64_t prow = 0, grow = 0;
+    int32_t beg[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 2; ++k) {
-        Sample& s = c->smp[k];
-        beg[k] = (int32_t)pb;
+        beg[k] = (int32_t)prow;
+        beg[3 + k] = (int32_t)grow;
+        const Sample& s = c->smp[k];
         if (!s.present) continue;
         for (size_t q = 0; q < s.seg_scheme.size(); ++q) {
             const int64_t n = s.seg_count[q];
             if (n == 0) continue;
-            const int64_t off = s.seg_begin[q];
-            const int scheme = force_unsmeared ? 0 : s.seg_scheme[q];
-            const bool mc = (k == PWA_MC) && !mc_as_data;
-            HIPCHK(c, launch_nll(c->desc.n_pc, mc, scheme, s.ev + off, s.ld, n, c->dS, c->dD, c->dC,
-                                 want_fx ? s.fx + off : nullptr, c->part + pb, noint, c->st));
-            pb += (n + kBlock - 1) / kBlock;
+            const int sb = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxShapeBlocks);
+            const int ab = grad ? (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxAdjBlocks) : 0;
+            segs.push_back({k, q, sb, ab});
+            prow += sb;
+            grow += grad ? sb + ab : 0;
         }
     }
-    beg[2] = (int32_t)pb;
+    beg[2] = (int32_t)prow;
+    beg[5] = (int32_t)grow;
+    prow = std::max<int64_t>(prow, 1);
+    if (prow > c->part_cap) {
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        HIPCHK(c, hipMalloc(&c->part, prow * sizeof(Partial)));
+        c->part_cap = prow;
+    }
+    if (grad && grow * NG > c->gpart_cap) {
+        if (c->gpart) (void)hipFree(c->gpart);
+        c->gpart = nullptr;
+        HIPCHK(c, hipMalloc(&c->gpart, std::max<int64_t>(grow, 1) * NG * sizeof(double)));
+        c->gpart_cap = grow * NG;
+    }
+    const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
+    c->nlaunch = 0;
+    HIPCHK(c, hipEventRecord(c->e0, c->st));
+    int64_t pr = 0, gr = 0;
+    for (const Seg& g : segs) {
+        Sample& s = c->smp[g.k];
+        const int64_t off = s.seg_begin[g.q], n = s.seg_count[g.q];
+        StageArgs a{};
+        a.ev = s.ev + off;
+        a.ld = s.ld;
+        a.n = n;
+        a.scheme = force_unsmeared ? 0 : s.seg_scheme[g.q];
+        a.mc = (g.k == PWA_MC) && !mc_as_data;
+        a.grad = grad;
+        a.km = c->hs.km_nclass > 0;
+        a.noint = noint;
+        a.S = c->dS;
+        a.D = c->dD;
+        a.cpl = c->dC;
+        a.inter = s.inter + off;
+        a.fx = want_fx ? s.fx + off : nullptr;
+        a.part = c->part + pr;
+        a.shape_blocks = g.sb;
+        a.adj_blocks = g.ab;
+        a.ncpl = c->ncpl;
+        a.nres = c->nres;
+        a.ng = NG;
+        a.gpart_shape = grad ? c->gpart + gr * NG : nullptr;
+        a.gpart_adj = grad ? c->gpart + (gr + g.sb) * NG : nullptr;
+        const int li = c->nlaunch < pwa_ctx::kMaxLaunch ? c->nlaunch++ : -1;
+        if (li >= 0) {
+            HIPCHK(c, hipEventRecord(c->le[li][0], c->st));
+            c->l_kind[li] = g.k;
+            c->l_scheme[li] = a.scheme;
+            c->l_n[li] = n;
+            c->l_grad[li] = grad ? 1 : 0;
+        }
+        HIPCHK(c, run_stages(c->desc.n_pc, a, c->st));
+        if (li >= 0) HIPCHK(c, hipEventRecord(c->le[li][1], c->st));
+        pr += g.sb;
+        gr += grad ? g.sb + g.ab : 0;
+    }
     HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, nullptr, c->dout, c->st));
+    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, c->dout, c->st));
+    if (grad) HIPCHK(c, launch_grad_reduce(c->gpart, c->dbeg + 3, NG, c->dgout, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (grad) HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, 2 * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
+    collect_times(c);
     return PWA_OK;
+}
+
+int run_eval(pwa_ctx* c, const double* params, bool want_fx, bool force_unsmeared, bool mc_as_data) {
+    if (!c->smp[PWA_DATA].present && !c->smp[PWA_MC].present) return fail(c, PWA_E_STATE, "no events uploaded");
+    return launch_all(c, params, false, want_fx, force_unsmeared, mc_as_data);
 }
 
 int copy_fx(pwa_ctx* c, int kind, double* out) {
@@ -352,7 +420,7 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipMalloc(&c->dS, sizeof(DevStatic)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dD, sizeof(DevDyn)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dC, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
-    if (bad(hipMalloc(&c->dbeg, 4 * sizeof(int32_t)))) return PWA_E_NOMEM;
+    if (bad(hipMalloc(&c->dbeg, 8 * sizeof(int32_t)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dout, 8 * sizeof(double)))) return PWA_E_NOMEM;
     c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams;
     if (bad(hipMalloc(&c->dgout, 2 * c->NG * sizeof(double)))) return PWA_E_NOMEM;
@@ -362,6 +430,9 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipHostMalloc(&c->hcpl, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
     if (bad(hipEventCreate(&c->e0))) return PWA_E_HIP;
     if (bad(hipEventCreate(&c->e1))) return PWA_E_HIP;
+    for (int q = 0; q < pwa_ctx::kMaxLaunch; ++q)
+        for (int e = 0; e < 2; ++e)
+            if (bad(hipEventCreate(&c->le[q][e]))) return PWA_E_HIP;
     if (bad(hipMemcpy(c->dS, &c->hs, sizeof(DevStatic), hipMemcpyHostToDevice))) return PWA_E_HIP;
     *out = c;
     return PWA_OK;
@@ -374,6 +445,7 @@ int pwa_destroy(pwa_ctx* c) {
     for (int k = 0; k < 2; ++k) {
         if (c->smp[k].ev) (void)hipFree(c->smp[k].ev);
         if (c->smp[k].fx) (void)hipFree(c->smp[k].fx);
+        if (c->smp[k].inter) (void)hipFree(c->smp[k].inter);
     }
     if (c->dS) hipFree(c->dS);
     if (c->dD) hipFree(c->dD);
@@ -389,6 +461,9 @@ int pwa_destroy(pwa_ctx* c) {
     if (c->hcpl) (void)hipHostFree(c->hcpl);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
+    for (int q = 0; q < pwa_ctx::kMaxLaunch; ++q)
+        for (int e = 0; e < 2; ++e)
+            if (c->le[q][e]) (void)hipEventDestroy(c->le[q][e]);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return PWA_OK;
@@ -409,6 +484,7 @@ int pwa_upload_events(pwa_ctx* c, int kind, int64_t n, const double* cols, int64
     Sample& s = c->smp[kind];
     if (s.ev) (void)hipFree(s.ev);
     if (s.fx) (void)hipFree(s.fx);
+    if (s.inter) (void)hipFree(s.inter);
     s = Sample();
     s.n = n;
     s.ld = std::max<int64_t>(((n + 255) / 256) * 256, 256);
@@ -439,6 +515,7 @@ int pwa_upload_events(pwa_ctx* c, int kind, int64_t n, const double* cols, int64
     double* draw = nullptr;
     HIPCHK(c, hipMalloc(&s.ev, (size_t)E_NPLANE * s.ld * sizeof(double)));
     HIPCHK(c, hipMalloc(&s.fx, (size_t)s.ld * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.inter, (size_t)inter_planes(c->desc.n_pc) * s.ld * sizeof(double)));
     HIPCHK(c, hipMemsetAsync(s.ev, 0, (size_t)E_NPLANE * s.ld * sizeof(double), c->st));
     if (n > 0) {
         HIPCHK(c, hipMalloc(&draw, raw.size() * sizeof(double)));
@@ -501,55 +578,11 @@ int pwa_eval_grad(pwa_ctx* c, const double* params, double* nll, double* grad) {
     if (!c || !params || !grad) return PWA_E_ARG;
     if (!c->smp[PWA_DATA].present || !c->smp[PWA_MC].present)
         return fail(c, PWA_E_STATE, "both data and MC must be uploaded before pwa_eval_grad");
-    int rc = upload_params(c, params);
+    if (c->hs.km_nclass > 1)
+        return fail(c, PWA_E_MODEL, "the analytic gradient supports one K-matrix J^P class per model");
+    int rc = launch_all(c, params, true, false, false, false);
     if (rc) return rc;
     const int NG = c->NG;
-    const int kMaxBlocks = 512;
-    // rows: one per grad block, data segments first
-    int64_t rows = 0;
-    for (int k = 0; k < 2; ++k)
-        for (int64_t cnt : c->smp[k].seg_count)
-            if (cnt > 0) rows += std::min<int64_t>((cnt + kBlock - 1) / kBlock, kMaxBlocks);
-    rows = std::max<int64_t>(rows, 1);
-    if (rows > c->part_cap) {
-        if (c->part) (void)hipFree(c->part);
-        c->part = nullptr;
-        HIPCHK(c, hipMalloc(&c->part, rows * sizeof(Partial)));
-        c->part_cap = rows;
-    }
-    if (rows * NG > c->gpart_cap) {
-        if (c->gpart) (void)hipFree(c->gpart);
-        c->gpart = nullptr;
-        HIPCHK(c, hipMalloc(&c->gpart, rows * NG * sizeof(double)));
-        c->gpart_cap = rows * NG;
-    }
-    const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
-    int32_t beg[3] = {0, 0, 0};
-    int64_t row = 0;
-    HIPCHK(c, hipEventRecord(c->e0, c->st));
-    for (int k = 0; k < 2; ++k) {
-        Sample& s = c->smp[k];
-        beg[k] = (int32_t)row;
-        for (size_t q = 0; q < s.seg_scheme.size(); ++q) {
-            const int64_t n = s.seg_count[q];
-            if (n == 0) continue;
-            const int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxBlocks);
-            HIPCHK(c, launch_grad(c->desc.n_pc, k == PWA_MC, s.seg_scheme[q], s.ev + s.seg_begin[q], s.ld, n, c->dS,
-                                  c->dD, c->dC, c->ncpl, c->nres, NG, blocks, c->gpart + row * NG, c->part + row,
-                                  noint, c->st));
-            row += blocks;
-        }
-    }
-    beg[2] = (int32_t)row;
-    HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, nullptr, c->dout, c->st));
-    HIPCHK(c, launch_grad_reduce(c->gpart, c->dbeg, NG, c->dgout, c->st));
-    HIPCHK(c, hipEventRecord(c->e1, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, 2 * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
     double I = 0.0;
     const double v = nll_from_sums(c, &I);
     if (nll) *nll = v;
@@ -643,7 +676,23 @@ int pwa_helicity_couplings(const pwa_model_desc* model, const double* params, do
 int pwa_last_timing(pwa_ctx* c, double* ms_total, double* ms_main) {
     if (!c) return PWA_E_ARG;
     if (ms_total) *ms_total = c->last_ms;
-    if (ms_main) *ms_main = c->last_ms;
+    double mx = 0.0;
+    for (int q = 0; q < c->nlaunch; ++q) mx = std::max(mx, (double)c->l_ms[q]);
+    if (ms_main) *ms_main = mx;
+    return PWA_OK;
+}
+
+int pwa_last_launches(pwa_ctx* c, int32_t cap, int32_t* n, double* ms, int32_t* kind, int32_t* scheme,
+                      int32_t* is_grad, int64_t* nevents) {
+    if (!c || !n) return PWA_E_ARG;
+    *n = c->nlaunch;
+    for (int q = 0; q < c->nlaunch && q < cap; ++q) {
+        if (ms) ms[q] = c->l_ms[q];
+        if (kind) kind[q] = c->l_kind[q];
+        if (scheme) scheme[q] = c->l_scheme[q];
+        if (is_grad) is_grad[q] = c->l_grad[q];
+        if (nevents) nevents[q] = c->l_n[q];
+    }
     return PWA_OK;
 }
 

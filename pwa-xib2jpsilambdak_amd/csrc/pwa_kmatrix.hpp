@@ -1,0 +1,417 @@
+// K-matrix lineshape of the Pc sector (host + device): the production
+// amplitudes of one J^P class at one mass, and the reverse-mode adjoint that
+// carries d/dMtr to the K-matrix couplings, background terms and pole masses.
+#pragma once
+#include "pwa_device.hpp"
+
+namespace pwa {
+
+PWA_HD double re_mul(const C64& a, const C64& b) { return a.re * b.re - a.im * b.im; }
+
+// ----------------------------------------------------------- K-matrix
+PWA_HD C64 rho_real(double s, double m1, double m2, int L, double ffb) {
+    const double rho2 = (s - (m1 + m2) * (m1 + m2)) * (s - (m1 - m2) * (m1 - m2));
+    const double rho = sqrt(fabs(rho2)) / s * fbw2(s, m1, m2, L, ffb);
+    return rho2 >= 0 ? C64(rho, 0.0) : C64(0.0, rho);
+}
+PWA_HD C64 rho_cmass(double s, double m1, double m2, double hw) {
+    const C64 a(m1 + m2, -hw), b(m1 - m2, hw);
+    const C64 p1 = a * a - C64(s, 0.0), p2 = b * b - C64(s, 0.0);
+    const C64 r = csqrt_branch(p1 * p2);
+    return C64(r.re / s, r.im / s);
+}
+
+// Production amplitudes of K-matrix class ik at mass mpp: Mtr[p] (poles) and
+// Mtr[np + c] (non-resonant channels), as Default_KMatrix
+// (src/cu_DPFPropogator.cu:588-813): T = row 0 of (D (1 - i K rho))^-1 with
+// K = sum_p g_p g_p^T / (m_p^2 - s) + f and D = prod_p (m_p^2 - s).
+static PWA_HD void kmatrix(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int ik, double mpp,
+                        C64 Mtr[8]) {
+    const int Lna = (ik >= 2) ? 1 : 0;
+    const double s = mpp * mpp;
+    C64 rho[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        rho[c] = S->km_cplx[ik][c] ? rho_cmass(s, S->km_ma[ik][c], S->km_mb[ik][c], S->km_hw[ik][c])
+                                   : rho_real(s, S->km_ma[ik][c], S->km_mb[ik][c], Lna, S->P.ffb);
+    const double* g = D->km[ik];
+    const double* f = D->km[ik] + 16;
+    const int np = D->km_npole[ik];
+    double den[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        const int r = D->km_pole_res[ik][p];
+        const double mfit = D->m0[r];
+        const int ch = D->km_pole_shift[ik][p];
+        double m02 = mfit * mfit;
+        if (ch > 0) {
+            const C64 rc = rho_cmass(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch]);
+            const double gc = g[p * 4 + ch];
+            const double m0 = sqrt(m02 - gc * gc * rc.im);
+            m02 = m0 * m0;
+        }
+        double d = m02 - s;
+        if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) d = 1e-3;
+        den[p] = d;
+    }
+    double Dall = 1.0, ms[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) Dall *= (p < np) ? den[p] : 1.0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double t = 1.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q != p && q < np) t *= den[q];
+        ms[p] = t;
+    }
+    double kma[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b) {
+            double v = f[a * 4 + b] * Dall;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (p < np) v += g[p * 4 + a] * (g[p * 4 + b] * ms[p]);
+            kma[a][b] = v;
+            kma[b][a] = v;
+        }
+    // A = KK^T, KK[a][b] = D delta_ab - i kma[a][b] rho[b]
+    C64 A[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            // A[a][b] = KK[b][a] = -i kma[b][a] rho[a] + D delta
+            const C64 t = rho[a] * kma[b][a];
+            A[a][b] = C64(t.im, -t.re);
+            if (a == b) A[a][b].re += Dall;
+        }
+    // Doolittle LU without pivoting, then solve A x = e0
+    C64 Lm[4][4], U[4][4], iU[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int col = a; col < 4; ++col) {
+            C64 u = A[a][col];
+#pragma unroll
+            for (int k = 0; k < a; ++k) u = u - Lm[a][k] * U[k][col];
+            U[a][col] = u;
+        }
+        iU[a] = cinv(U[a][a]);
+#pragma unroll
+        for (int row = a + 1; row < 4; ++row) {
+            C64 l = A[row][a];
+#pragma unroll
+            for (int k = 0; k < a; ++k) l = l - Lm[row][k] * U[k][a];
+            Lm[row][a] = l * iU[a];
+        }
+    }
+    C64 y[4], T[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        C64 v = (a == 0) ? C64(1.0, 0.0) : C64(0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < a; ++b) v = v - Lm[a][b] * y[b];
+        y[a] = v;
+    }
+#pragma unroll
+    for (int a = 3; a >= 0; --a) {
+        C64 v = y[a];
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b) v = v - U[a][b] * T[b];
+        T[a] = iU[a] * v;
+    }
+    const double na = Lna ? fbw2(s, S->km_ma[ik][0], S->km_mb[ik][0], Lna, S->P.ffb) : 1.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) Mtr[p] = C64(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        C64 acc;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * ms[p], T[c]);
+        Mtr[p] = acc * na;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const C64 v = T[c] * (Dall * na);
+        // Mtr[np + c] = v, written with compile-time indices only
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == np + c) Mtr[q] = v;
+    }
+}
+
+// ------------------------------------------------------ K-matrix adjoint
+// Forward state of one class at one mass, kept for the reverse pass.
+struct KMState {
+    C64 rho[4];
+    double den[4];
+    bool clamped[4];
+    double ms[4];
+    double Dall;
+    double kma[4][4];
+    C64 Lm[4][4], U[4][4], iU[4];
+    C64 T[4];
+    double na;
+    int np;
+    double mfit[4], rcim[4], drcim[4];  // pole mass, Im rho_shift, d Im rho_shift / d mfit
+    int ch[4];
+};
+
+// rho for the complex-mass shift channel and d(Im rho)/dM2 (branch as in csqrt_branch)
+PWA_HD void rho_cmass_d(double M2, double m1, double m2, double hw, double& im, double& dim) {
+    const C64 a(m1 + m2, -hw), b(m1 - m2, hw);
+    const C64 c1 = a * a, c2 = b * b;
+    const C64 P = (c1 - C64(M2, 0.0)) * (c2 - C64(M2, 0.0));
+    const C64 r = csqrt_branch(P);
+    im = r.im / M2;
+    if (r.re == 0.0 && r.im == 0.0) {
+        dim = 0.0;
+        return;
+    }
+    const C64 dP = C64(2.0 * M2, 0.0) - c1 - c2;
+    const C64 dr = dP * cinv(r * 2.0);  // d sqrt(P) / dM2
+    dim = dr.im / M2 - r.im / (M2 * M2);
+}
+
+PWA_HD void kmatrix_state(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int ik, double mpp,
+                              KMState& st, C64 Mtr[8]) {
+    const int Lna = (ik >= 2) ? 1 : 0;
+    const double s = mpp * mpp;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        st.rho[c] = S->km_cplx[ik][c] ? rho_cmass(s, S->km_ma[ik][c], S->km_mb[ik][c], S->km_hw[ik][c])
+                                      : rho_real(s, S->km_ma[ik][c], S->km_mb[ik][c], Lna, S->P.ffb);
+    const double* g = D->km[ik];
+    const double* f = D->km[ik] + 16;
+    const int np = D->km_npole[ik];
+    st.np = np;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        st.den[p] = 1.0;
+        st.clamped[p] = true;
+        st.ch[p] = 0;
+        st.mfit[p] = 0.0;
+        st.rcim[p] = 0.0;
+        st.drcim[p] = 0.0;
+        if (p >= np) continue;
+        const int r = D->km_pole_res[ik][p];
+        const double mfit = D->m0[r];
+        const int ch = D->km_pole_shift[ik][p];
+        st.mfit[p] = mfit;
+        st.ch[p] = ch;
+        double m02 = mfit * mfit;
+        if (ch > 0) {
+            double im, dim;
+            rho_cmass_d(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch], im, dim);
+            const double gc = g[p * 4 + ch];
+            const double m0 = sqrt(m02 - gc * gc * im);
+            st.rcim[p] = im;
+            st.drcim[p] = dim * 2.0 * mfit;
+            m02 = m0 * m0;
+        }
+        double d = m02 - s;
+        bool cl = false;
+        if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) {
+            d = 1e-3;
+            cl = true;
+        }
+        st.den[p] = d;
+        st.clamped[p] = cl;
+    }
+    double Dall = 1.0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) Dall *= (p < np) ? st.den[p] : 1.0;
+    st.Dall = Dall;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double t = 1.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q != p && q < np) t *= st.den[q];
+        st.ms[p] = t;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = x; y < 4; ++y) {
+            double v = f[x * 4 + y] * Dall;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (p < np) v += g[p * 4 + x] * (g[p * 4 + y] * st.ms[p]);
+            st.kma[x][y] = v;
+            st.kma[y][x] = v;
+        }
+    C64 A[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const C64 t = st.rho[x] * st.kma[y][x];
+            A[x][y] = C64(t.im, -t.re);
+            if (x == y) A[x][y].re += Dall;
+        }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+        for (int col = x; col < 4; ++col) {
+            C64 u = A[x][col];
+#pragma unroll
+            for (int k = 0; k < x; ++k) u = u - st.Lm[x][k] * st.U[k][col];
+            st.U[x][col] = u;
+        }
+        st.iU[x] = cinv(st.U[x][x]);
+#pragma unroll
+        for (int row = x + 1; row < 4; ++row) {
+            C64 l = A[row][x];
+#pragma unroll
+            for (int k = 0; k < x; ++k) l = l - st.Lm[row][k] * st.U[k][x];
+            st.Lm[row][x] = l * st.iU[x];
+        }
+    }
+    C64 y[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        C64 v = (x == 0) ? C64(1.0, 0.0) : C64(0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < x; ++b) v = v - st.Lm[x][b] * y[b];
+        y[x] = v;
+    }
+#pragma unroll
+    for (int x = 3; x >= 0; --x) {
+        C64 v = y[x];
+#pragma unroll
+        for (int b = x + 1; b < 4; ++b) v = v - st.U[x][b] * st.T[b];
+        st.T[x] = st.iU[x] * v;
+    }
+    st.na = Lna ? fbw2(s, S->km_ma[ik][0], S->km_mb[ik][0], Lna, S->P.ffb) : 1.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) Mtr[p] = C64(0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        C64 acc;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * st.ms[p], st.T[c]);
+        Mtr[p] = acc * st.na;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const C64 v = st.T[c] * (Dall * st.na);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == np + c) Mtr[q] = v;
+    }
+}
+
+// Reverse pass: Abar[q] is the adjoint of Mtr[q] (delta F = Re sum_q Abar_q dMtr_q).
+// Accumulates Re-parts into gb[16] (g[p][c]), fb[16] (f[x][y], x <= y), mb[4] (pole masses).
+PWA_HD void kmatrix_adjoint(const DevDyn* __restrict__ D, int ik, const KMState& st, const C64 Abar[8],
+                                double gb[16], double fb[16], double mb[4]) {
+    const double* g = D->km[ik];
+    const double* f = D->km[ik] + 16;
+    const int np = st.np;
+    C64 Tb[4];
+    C64 Db(0.0, 0.0), msb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        C64 an;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (q == np + c) an = Abar[q];
+        Tb[c] = an * (st.na * st.Dall);
+        cmac(Db, an * st.na, st.T[c]);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        const C64 ap = Abar[p] * st.na;
+        C64 gt;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            rmac(Tb[c], g[p * 4 + c] * st.ms[p], ap);
+            gb[p * 4 + c] += re_mul(ap, st.T[c]) * st.ms[p];
+            rmac(gt, g[p * 4 + c], st.T[c]);
+        }
+        msb[p] = ap * gt;
+    }
+    // lambda: A^T lambda = Tb with A = L U  ->  U^T y = Tb, L^T lambda = y
+    C64 yv[4], lam[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        C64 v = Tb[x];
+#pragma unroll
+        for (int b = 0; b < x; ++b) v = v - st.U[b][x] * yv[b];
+        yv[x] = v * st.iU[x];
+    }
+#pragma unroll
+    for (int x = 3; x >= 0; --x) {
+        C64 v = yv[x];
+#pragma unroll
+        for (int b = x + 1; b < 4; ++b) v = v - st.Lm[b][x] * lam[b];
+        lam[x] = v;
+    }
+    // Abar_matrix[x][y] = -lam_x T_y; A[x][y] = Dall delta - i kma[y][x] rho[x]
+    // kma adjoint (row y, col x): kbar[y][x] = -i rho[x] * Abar_matrix[x][y] = i rho[x] lam_x T_y
+#pragma unroll
+    for (int x = 0; x < 4; ++x) Db = Db - lam[x] * st.T[x];
+    C64 ub[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const C64 t = st.rho[y] * lam[y] * st.T[x];  // kbar[x][y] = i rho[y] lam_y T_x
+            ub[x][y] = C64(-t.im, t.re);
+        }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = x; y < 4; ++y) {
+            const C64 u = (x == y) ? ub[x][x] : ub[x][y] + ub[y][x];
+            fb[x * 4 + y] += u.re * st.Dall;
+            rmac(Db, f[x * 4 + y], u);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                if (p >= np) continue;
+                gb[p * 4 + x] += u.re * g[p * 4 + y] * st.ms[p];
+                gb[p * 4 + y] += u.re * g[p * 4 + x] * st.ms[p];
+                rmac(msb[p], g[p * 4 + x] * g[p * 4 + y], u);
+            }
+        }
+    // D = prod den, ms_p = prod_{q != p} den
+    double denb[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q >= np) continue;
+        double v = Db.re * st.ms[q];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            if (p >= np || p == q) continue;
+            double t = 1.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (r < np && r != p && r != q) t *= st.den[r];
+            v += msb[p].re * t;
+        }
+        denb[q] = st.clamped[q] ? 0.0 : v;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        if (p >= np) continue;
+        const int ch = st.ch[p];
+        if (ch > 0) {
+            const double gc = g[p * 4 + ch];
+            mb[p] += denb[p] * (2.0 * st.mfit[p] - gc * gc * st.drcim[p]);
+            gb[p * 4 + ch] += denb[p] * (-2.0 * gc * st.rcim[p]);
+        } else {
+            mb[p] += denb[p] * 2.0 * st.mfit[p];
+        }
+    }
+}
+
+
+} // namespace pwa

@@ -252,7 +252,7 @@ PWA_HD C64 csqrt_branch(C64 x) {
     double cosA = x.re / radius;
     double re = sqrt(radius * (cosA + 1.0) / 2.0);
     double im = sqrt(radius * (1.0 - cosA) / 2.0);
-    if (signbit(x.im) && fabs(x.im / x.re) > 0.001) im = -im;
+    if (__builtin_signbit(x.im) && fabs(x.im / x.re) > 0.001) im = -im;
     return C64(re, im);
 }
 

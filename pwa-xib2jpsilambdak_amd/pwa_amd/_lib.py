@@ -45,6 +45,7 @@ def lib():
             "pwa_alpha": (C.c_int, [vp, _dp]),
             "pwa_amp2": (C.c_int, [vp, C.c_int, _dp, _dp]),
             "pwa_last_timing": (C.c_int, [vp, _dp, _dp]),
+            "pwa_last_launches": (C.c_int, [vp, C.c_int32, _ip, _dp, _ip, _ip, _ip, C.POINTER(C.c_int64)]),
             "pwa_generate_phsp": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double,
                                             C.c_double, C.c_double, _dp]),
             "pwa_generate_phsp_p4": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double,
@@ -194,6 +195,19 @@ class Context:
         out = np.zeros(self.n[kind])
         check(lib().pwa_amp2(self._h, kind, dptr(p), dptr(out)), self._h, "pwa_amp2")
         return out
+
+    def last_launches(self):
+        """Per-launch (ms, kind, scheme, is_grad, n_events) of the last evaluation."""
+        cap = 32
+        n = C.c_int32()
+        ms = np.zeros(cap)
+        kind, sch, gr = (np.zeros(cap, np.int32) for _ in range(3))
+        ne = np.zeros(cap, np.int64)
+        check(lib().pwa_last_launches(self._h, cap, C.byref(n), dptr(ms), iptr(kind), iptr(sch), iptr(gr),
+                                      ne.ctypes.data_as(C.POINTER(C.c_int64))), self._h, "pwa_last_launches")
+        k = min(n.value, cap)
+        return [dict(ms=float(ms[q]), kind=int(kind[q]), scheme=int(sch[q]), grad=bool(gr[q]), n=int(ne[q]))
+                for q in range(k)]
 
     def last_timing(self):
         t = C.c_double()

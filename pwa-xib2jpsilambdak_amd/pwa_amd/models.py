@@ -274,7 +274,7 @@ def config_kmatrix(seed=3, nbins=40) -> Model:
     (all 10 entries, PWA_F_KM_FULL_F); production through the four
     non-resonant members' couplings."""
     rng = np.random.default_rng(seed)
-    d = base_desc(flags=abi.PWA_F_KM_CLAMP)
+    d = base_desc()  # no pole clamp: the likelihood stays continuous in the pole masses
     _nominal_smearing(d, nbins)
     chans = [(M_P, M_JPSI, 0.0, 0), (D0, SIGC_P, SIGC_P_HW, 1), (DST0, SIGC_P, SIGC_P_HW, 1),
              (DM, SIGC_PP, SIGC_PP_HW, 1)]

@@ -1,0 +1,37 @@
+#include <cstdio>
+#include <random>
+#include "pwa_kmatrix.hpp"
+using namespace pwa;
+int main() {
+    DevStatic S{}; DevDyn D{};
+    S.P = Masses{5.6196, 0.938272, 0.493677, 3.0969, 3.0, 3.0};
+    S.flags = 0;
+    double ma[4] = {0.938272, 1.86484, 2.00685, 1.86966}, mb[4] = {3.0969, 2.45265, 2.45265, 2.45397}, hw[4] = {0, 0.00115, 0.00115, 0.000945};
+    for (int c = 0; c < 4; ++c) { S.km_ma[0][c] = ma[c]; S.km_mb[0][c] = mb[c]; S.km_hw[0][c] = hw[c]; S.km_cplx[0][c] = c > 0; }
+    for (int c = 1; c < 4; ++c) { S.km_shift_ma[c] = ma[c]; S.km_shift_mb[c] = mb[c]; S.km_shift_hw[c] = hw[c]; }
+    std::mt19937_64 rng(5); std::normal_distribution<double> N(0, 0.5);
+    D.km_npole[0] = 2; D.km_pole_res[0][0] = 0; D.km_pole_res[0][1] = 1; D.km_pole_shift[0][0] = 1; D.km_pole_shift[0][1] = 2;
+    D.m0[0] = 4.31; D.m0[1] = 4.47;
+    for (int q = 0; q < 8; ++q) D.km[0][q] = N(rng);
+    for (int x = 0; x < 4; ++x) for (int y = x; y < 4; ++y) { double v = N(rng); D.km[0][16 + x * 4 + y] = v; D.km[0][16 + y * 4 + x] = v; }
+    C64 Ab[8]; for (int q = 0; q < 6; ++q) Ab[q] = C64(N(rng), N(rng));
+    double worst = 0;
+    for (double mpp : {4.2, 4.33, 4.46, 4.55}) {
+        KMState st; C64 M0[8];
+        kmatrix_state(&S, &D, 0, mpp, st, M0);
+        C64 Mk[8]; kmatrix(&S, &D, 0, mpp, Mk);
+        for (int q = 0; q < 6; ++q) worst = fmax(worst, hypot(Mk[q].re - M0[q].re, Mk[q].im - M0[q].im));
+        double gb[16] = {0}, fb[16] = {0}, mbp[4] = {0};
+        kmatrix_adjoint(&D, 0, st, Ab, gb, fb, mbp);
+        auto F = [&](DevDyn& d2) { C64 M[8]; kmatrix(&S, &d2, 0, mpp, M); double f = 0; for (int q = 0; q < 6; ++q) f += re_mul(Ab[q], M[q]); return f; };
+        auto fd = [&](double* x) { double h = 1e-6 * fmax(1.0, fabs(*x)); double x0 = *x; DevDyn d2 = D; 
+            double* p = (double*)((char*)&d2 + ((char*)x - (char*)&D)); *p = x0 + h; double fp = F(d2); *p = x0 - h; double fm = F(d2); return (fp - fm) / (2 * h); };
+        for (int q = 0; q < 8; ++q) { double a = gb[q], b = fd(&D.km[0][q]); printf("m=%.2f g[%d] an=% .8e fd=% .8e\n", mpp, q, a, b); worst = fmax(worst, fabs(a - b) / fmax(1e-3, fabs(b))); }
+        for (int x = 0; x < 4; ++x) for (int y = x; y < 4; ++y) {
+            DevDyn d2 = D; double h = 1e-6; d2.km[0][16 + x*4+y] += h; d2.km[0][16 + y*4+x] = d2.km[0][16 + x*4+y]; double fp = F(d2);
+            d2.km[0][16 + x*4+y] -= 2*h; d2.km[0][16 + y*4+x] = d2.km[0][16 + x*4+y]; double fm = F(d2); double b = (fp - fm) / (2*h);
+            printf("m=%.2f f[%d][%d] an=% .8e fd=% .8e\n", mpp, x, y, fb[x*4+y], b); worst = fmax(worst, fabs(fb[x*4+y] - b) / fmax(1e-3, fabs(b))); }
+        for (int p = 0; p < 2; ++p) { double a = mbp[p], b = fd(&D.m0[p]); printf("m=%.2f mp[%d] an=% .8e fd=% .8e\n", mpp, p, a, b); worst = fmax(worst, fabs(a - b) / fmax(1e-3, fabs(b))); }
+    }
+    printf("WORST rel %.3e\n", worst);
+}
