@@ -10,34 +10,73 @@ namespace pwa {
 
 using D2 = Dual<2>;
 
-constexpr int kStage = 16;  // components per staging tile
+// Per-event gradient contributions are staged in LDS as rows of 64 lanes
+// (one row per gradient component, one column per event of the wave), then
+// reduced 16 rows at a time into the wave's accumulator.  All pointers are
+// typed as LDS (address space 3) so every access is a ds_* instruction.
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
+constexpr int kRows = 16;  // staged rows per wave
+constexpr int kLd = 65;    // row stride (odd: the transposed reads hit distinct banks)
+
+// LDS bytes of a gradient kernel block (4 waves): stage, row indices, accumulators.
+__host__ __device__ constexpr int grad_lds_doubles(int NGp) { return 4 * kRows * kLd + 4 * NGp + 4 * kRows / 2; }
 
 struct Emitter {
-    double* stage;   // [kStage][64]
-    int32_t* sidx;   // [kStage]
-    double* acc;     // [NG]
-    int cnt;
+    lds_f64* stage;  // [kRows][kLd]
+    lds_i32* sidx;   // [kRows] gradient index of each staged row
+    lds_f64* acc;    // [NGp] this wave's accumulator
+    int cnt;         // staged rows (uniform)
     int lane;
+
+    __device__ __forceinline__ static Emitter make(double* lds_raw, int NGp) {
+        lds_f64* l = (lds_f64*)lds_raw;
+        const int wave = threadIdx.x >> 6;
+        Emitter e;
+        e.stage = l + wave * (kRows * kLd);
+        e.acc = l + 4 * kRows * kLd + wave * NGp;
+        e.sidx = (lds_i32*)(l + 4 * kRows * kLd + 4 * NGp) + wave * kRows;
+        e.cnt = 0;
+        e.lane = threadIdx.x & 63;
+        for (int q = e.lane; q < NGp; q += 64) e.acc[q] = 0.0;
+        return e;
+    }
+    // Sum of the four wave accumulators at gradient index q (after a block barrier).
+    __device__ __forceinline__ static double block_sum(double* lds_raw, int NGp, int q) {
+        const lds_f64* a = (const lds_f64*)lds_raw + 4 * kRows * kLd + q;
+        return ((a[0] + a[NGp]) + a[2 * NGp]) + a[3 * NGp];
+    }
     __device__ __forceinline__ void flush() {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const int c = lane & (kStage - 1), q = lane / kStage;  // 4 lanes per component, 16 events each
+        // lane (q, c): row c of the tile, events 16 q .. 16 q + 15
+        const int c = lane & 15, q = lane >> 4;
         double s = 0.0;
-        if (c < cnt) {
+        const bool live = c < cnt;
+        if (live) {
+            const lds_f64* src = stage + c * kLd + q * 16;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) s += stage[c * 64 + q * 16 + e];
+            for (int e = 0; e < 16; ++e) s += src[e];
         }
         s += __shfl_xor(s, 16);
         s += __shfl_xor(s, 32);
-        if (q == 0 && c < cnt) atomicAdd(&acc[sidx[c]], s);  // a tile may hold one index twice
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (q == 0 && live)  // rows of one tile may carry the same index
+            __hip_atomic_fetch_add(acc + sidx[c], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
     }
-    __device__ __forceinline__ void emit(int idx, double v) {
-        stage[cnt * 64 + lane] = v;
-        if (lane == 0) sidx[cnt] = idx;
-        if (++cnt == kStage) flush();
+    // Make room for n <= kRows rows.
+    __device__ __forceinline__ void begin(int n) {
+        if (cnt + n > kRows) flush();
+    }
+    // Value of staged row cnt + j.
+    __device__ __forceinline__ void put(int j, double v) { stage[(cnt + j) * kLd + lane] = v; }
+    // Rows cnt .. cnt + n - 1 carry gradient indices base .. base + n - 1.
+    __device__ __forceinline__ void commit(int n, int base) {
+        if (lane < n) sidx[cnt + lane] = base + lane;
+        cnt += n;
     }
 };
 
@@ -55,8 +94,80 @@ struct GradLayout {
 __device__ __forceinline__ C64 cval(const Cx<D2>& x) { return C64(x.re.v, x.im.v); }
 __device__ __forceinline__ C64 cder(const Cx<D2>& x, int k) { return C64(x.re.d[k], x.im.d[k]); }
 
+// ------------------------------------- lineshape x vertex with derivatives
+// Value and (m0, Gamma0) derivatives of the Lambda* lineshape x vertex term.
+__device__ __forceinline__ void lambda_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                               const ChainKin& kin, LsEv& e, double mkp, C64& v, C64& d0,
+                                               C64& d1) {
+    const LsConst& lc = D->ls[r];
+    if (lc.kind != LS_GENERIC) {
+        ls_event_cached(S->P, kin, lc, e);
+        ls_grad(lc, kin, e, v, d0, d1);
+        return;
+    }
+    const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
+    Cx<D2> Ls(D2(1.0), D2(0.0));
+    if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<D2>(S->P, m0, g0, mkp, S->L[r]);
+    Ls = Ls * vertex_lambda<D2>(S->P, S->spin2[r], mkp, m0);
+    v = cval(Ls);
+    d0 = cder(Ls, 0);
+    d1 = cder(Ls, 1);
+}
+
+// Same for a Zcs resonance.
+__device__ __forceinline__ void zcs_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                            const ChainKin& kin, LsEv& e, double mjk, C64& v, C64& d0, C64& d1) {
+    const LsConst& lc = D->ls[r];
+    if (lc.kind != LS_GENERIC) {
+        ls_event_cached(S->P, kin, lc, e);
+        ls_grad(lc, kin, e, v, d0, d1);
+        return;
+    }
+    const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
+    Cx<D2> Ls = bw_twobody<D2>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
+    Ls = Ls * vertex_zcs<D2>(S->P, S->spin2[r], mjk, m0);
+    v = cval(Ls);
+    d0 = cder(Ls, 0);
+    d1 = cder(Ls, 1);
+}
+
+// Pc resonance rr at sub-bin mass m: L = Mt x vertex (K-matrix or
+// non-resonant members, Mt = K-matrix amplitude or 1) or Breit-Wigner x
+// vertex, its (m0, Gamma0) derivatives and the vertex factor Vx.
+__device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int rr,
+                                           const ChainKin& kin, LsEv& e, double m, C64 Mt, C64& L, C64& d0,
+                                           C64& d1, double& Vx) {
+    const LsConst& lc = D->ls[rr];
+    if (lc.kind != LS_GENERIC) {
+        ls_event_cached(S->P, kin, lc, e);
+        C64 v, a0, a1;
+        ls_grad(lc, kin, e, v, a0, a1);
+        if (lc.kind == LS_BW) {
+            L = v;
+            d0 = a0;
+            d1 = a1;
+            Vx = 0.0;
+        } else {
+            L = Mt * v.re;
+            d0 = Mt * a0.re;
+            d1 = Mt * a1.re;
+            Vx = v.re;
+        }
+        return;
+    }
+    const D2 m0 = D2::var(D->m0[rr], 0), g0 = D2::var(D->g0[rr], 1);
+    Cx<D2> Ls(D2(Mt.re), D2(Mt.im));
+    if (S->type[rr] == PWA_BREITWIGNER) Ls = bw_twobody<D2>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, g0, m, S->L[rr]);
+    const D2 Vv = vertex_pc<D2>(S->P, S->spin2[rr], m, m0);
+    Vx = Vv.v;
+    Ls = Ls * Vv;
+    L = cval(Ls);
+    d0 = cder(Ls, 0);
+    d1 = cder(Ls, 1);
+}
+
 // ---------------------------------------------------------- Lambda* adjoint
-static __device__ void lambda_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+__device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld, int64_t i,
                                double mkp, const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
     const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
@@ -86,6 +197,9 @@ static __device__ void lambda_adjoint(const DevStatic* __restrict__ S, const Dev
     double cp[10], sp[10];
     powers(cL, cp, 9);
     powers(sL, sp, 9);
+    const ChainKin kin = chain_kin(S->P, CH_LAMBDA, mkp);
+    LsEv ev_ls;
+    ev_ls.key = -1;
     for (int g = 0; g < 5; ++g) {
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
@@ -100,27 +214,30 @@ static __device__ void lambda_adjoint(const DevStatic* __restrict__ S, const Dev
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
-            const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
-            Cx<D2> Ls(D2(1.0), D2(0.0));
-            if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<D2>(S->P, m0, g0, mkp, S->L[r]);
-            Ls = Ls * vertex_lambda<D2>(S->P, S->spin2[r], mkp, m0);
-            const C64 Lv = cval(Ls);
+            C64 Lv, dL0, dL1;
+            lambda_ls_grad(S, D, r, kin, ev_ls, mkp, Lv, dL0, dL1);
+            Lv = Lv * a2;
             const C64* c = cpl + S->ampcount[r];
+            const int base = G.cpl(S->ampcount[r]);
             C64 R;
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
+            for (int k = 0; k < 2; ++k) {
+                em.begin(12);
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
                     if (m >= nm) continue;
                     const C64 dq = Q[k][m] * dl[k][m];
                     const C64 z = Lv * dq;
-                    const int j = S->ampcount[r] + k * nm + m;
-                    em.emit(G.cpl(j), a2 * z.re);
-                    em.emit(G.cpl(j) + 1, -a2 * z.im);
+                    em.put(2 * m, z.re);
+                    em.put(2 * m + 1, -z.im);
                     cmac(R, c[k * nm + m], dq);
                 }
-            em.emit(G.res(r), a2 * re_mul(cder(Ls, 0), R));
-            em.emit(G.res(r) + 1, a2 * re_mul(cder(Ls, 1), R));
+                em.commit(2 * nm, base + 2 * k * nm);
+            }
+            em.begin(2);
+            em.put(0, a2 * re_mul(dL0, R));
+            em.put(1, a2 * re_mul(dL1, R));
+            em.commit(2, G.res(r));
         }
     }
 }
@@ -149,7 +266,7 @@ __device__ __forceinline__ void chain_adjoint(const ChainAng& a, const C64 W[8],
 }
 
 // ------------------------------------------------------------- Zcs adjoint
-static __device__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+__device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                             const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld, int64_t i,
                             double mjk, const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
     const ChainAng a = load_chain(ev, ld, i, E_X_CB);
@@ -163,6 +280,9 @@ static __device__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn
     for (int h = 0; h < 8; ++h) cY[h] = conj(Y[h]);
     C64 T[2][3][2];
     chain_adjoint(a, cY, T);
+    const ChainKin kin = chain_kin(S->P, CH_ZCS, mjk);
+    LsEv ev_ls;
+    ev_ls.key = -1;
     for (int q = 0; q < S->nX; ++q) {
         const int r = S->nR + S->nZ + q;
         const int J2 = S->spin2[r];
@@ -173,10 +293,9 @@ static __device__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn
         case 4: dx_table<4>(cp, sp, dx); break;
         default: dx_table<6>(cp, sp, dx); break;
         }
-        const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
-        Cx<D2> Ls = bw_twobody<D2>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
-        Ls = Ls * vertex_zcs<D2>(S->P, J2, mjk, m0);
-        const C64 Lv = cval(Ls);
+        C64 Lv, dL0, dL1;
+        zcs_ls_grad(S, D, r, kin, ev_ls, mjk, Lv, dL0, dL1);
+        Lv = Lv * a2;
         const C64* c = cpl + S->ampcount[r];
         C64 R;
         int n = 0;
@@ -185,6 +304,8 @@ static __device__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn
 #pragma unroll
             for (int ps = 0; ps < 3; ++ps) {
                 if (2 * abs(ps - 1) > J2) continue;
+                em.begin(8);
+                const int n0 = n;
 #pragma unroll
                 for (int ip = 0; ip < 2; ++ip)
 #pragma unroll
@@ -194,20 +315,22 @@ static __device__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn
                         const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dx[hz2 / 2 + 1][(1 - ps) + 1];
                         const C64 t = T[k][ps][iz] * w;
                         const C64 z = Lv * t;
-                        const int j = S->ampcount[r] + n;
-                        em.emit(G.cpl(j), a2 * z.re);
-                        em.emit(G.cpl(j) + 1, -a2 * z.im);
+                        em.put(2 * (n - n0), z.re);
+                        em.put(2 * (n - n0) + 1, -z.im);
                         cmac(R, c[n], t);
                         ++n;
                     }
+                em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
             }
-        em.emit(G.res(r), a2 * re_mul(cder(Ls, 0), R));
-        em.emit(G.res(r) + 1, a2 * re_mul(cder(Ls, 1), R));
+        em.begin(2);
+        em.put(0, a2 * re_mul(dL0, R));
+        em.put(1, a2 * re_mul(dL1, R));
+        em.commit(2, G.res(r));
     }
 }
 
 // ---------------------------------------------------- Pc coupling adjoint
-static __device__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int r, const ChainAng& a, const double* cp,
+__device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int r, const ChainAng& a, const double* cp,
                                     const double* sp, double cP, double sP, const C64 U[8], double a2,
                                     const GradLayout& G, Emitter& em) {
     const int J2 = S->spin2[r];
@@ -224,20 +347,23 @@ static __device__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int 
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
-        for (int ps = 0; ps < 3; ++ps)
+        for (int ps = 0; ps < 3; ++ps) {
+            em.begin(8);
+            const int n0 = n;
 #pragma unroll
             for (int iz = 0; iz < 2; ++iz)
 #pragma unroll
                 for (int ip = 0; ip < 2; ++ip) {
                     const int n2 = 2 * (ps - 1) - (2 * ip - 1);
                     if (n2 > J2 || n2 < -J2) continue;
-                    const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
+                    const double w = a2 * dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
                     const C64 z = T[k][ps][iz] * w;
-                    const int j = S->ampcount[r] + n;
-                    em.emit(G.cpl(j), a2 * z.re);
-                    em.emit(G.cpl(j) + 1, -a2 * z.im);
+                    em.put(2 * (n - n0), z.re);
+                    em.put(2 * (n - n0) + 1, -z.im);
                     ++n;
                 }
+            em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
+        }
 }
 
 } // namespace pwa

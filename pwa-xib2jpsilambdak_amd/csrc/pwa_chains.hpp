@@ -64,6 +64,35 @@ __device__ __forceinline__ C64 cpow_int(C64 e, int n) {
     return n > 0 ? e : conj(e);
 }
 
+// ------------------------------------------------ lineshape x vertex values
+// Lambda* resonance r at m(K p): factorised form when available, else the
+// direct evaluation (bw_lambda x vertex_lambda).
+__device__ __forceinline__ C64 lambda_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                         const ChainKin& kin, LsEv& e, double mkp) {
+    const LsConst& lc = D->ls[r];
+    if (lc.kind != LS_GENERIC) {
+        ls_event_cached(S->P, kin, lc, e);
+        return ls_value(lc, kin, e);
+    }
+    const double m0 = D->m0[r], g0 = D->g0[r];
+    C64 Ls(1.0, 0.0);
+    if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<double>(S->P, m0, g0, mkp, S->L[r]);
+    return Ls * vertex_lambda<double>(S->P, S->spin2[r], mkp, m0);
+}
+
+// Zcs resonance r at m(J/psi K).
+__device__ __forceinline__ C64 zcs_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                      const ChainKin& kin, LsEv& e, double mjk) {
+    const LsConst& lc = D->ls[r];
+    if (lc.kind != LS_GENERIC) {
+        ls_event_cached(S->P, kin, lc, e);
+        return ls_value(lc, kin, e);
+    }
+    const double m0 = D->m0[r], g0 = D->g0[r];
+    const C64 Ls = bw_twobody<double>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
+    return Ls * vertex_zcs<double>(S->P, S->spin2[r], mjk, m0);
+}
+
 // ------------------------------------------------------ Lambda* chain
 // H[4i+2j+k] += sum_r L_r sum_m c_{r,k,m} DL_J[k][m] A[i][j][m]   (GetAmpLz, src/kernel_calEva.cu:85-190)
 __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
@@ -77,6 +106,9 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
     double cp[10], sp[10];
     powers(cL, cp, 9);
     powers(sL, sp, 9);
+    const ChainKin kin = chain_kin(S->P, CH_LAMBDA, mkp);
+    LsEv ev_ls;
+    ev_ls.key = -1;
 
     C64 G[2][6];
     for (int g = 0; g < 5; ++g) {
@@ -93,10 +125,7 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
-            const double m0 = D->m0[r], g0 = D->g0[r];
-            C64 Ls(1.0, 0.0);
-            if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<double>(S->P, m0, g0, mkp, S->L[r]);
-            Ls = Ls * vertex_lambda<double>(S->P, S->spin2[r], mkp, m0);
+            const C64 Ls = lambda_ls(S, D, r, kin, ev_ls, mkp);
             const C64* c = cpl + S->ampcount[r];
 #pragma unroll
             for (int k = 0; k < 2; ++k)
@@ -223,6 +252,9 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
     double cp[8], sp[8];
     powers(cX, cp, 7);
     powers(sX, sp, 7);
+    const ChainKin kin = chain_kin(S->P, CH_ZCS, mjk);
+    LsEv ev_ls;
+    ev_ls.key = -1;
     for (int q = 0; q < S->nX; ++q) {
         const int r = S->nR + S->nZ + q;
         const int J2 = S->spin2[r];
@@ -254,9 +286,7 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
             }
         C64 Sx[8];
         chain_contract(a, V, Sx);
-        const double m0 = D->m0[r], g0 = D->g0[r];
-        C64 Ls = bw_twobody<double>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
-        Ls = Ls * vertex_zcs<double>(S->P, J2, mjk, m0);
+        const C64 Ls = zcs_ls(S, D, r, kin, ev_ls, mjk);
 #pragma unroll
         for (int h = 0; h < 8; ++h) cmac(H[h], Ls, Sx[h]);
     }
@@ -286,9 +316,19 @@ __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, c
         for (int r = 0; r < NZ; ++r)
             if (S->ik[nR + r] == ik) L[r] = select8(Mtr, S->ikm[nR + r]);
     }
+    const ChainKin kin = chain_kin(S->P, CH_PC, m);
+    LsEv e;
+    e.key = -1;
 #pragma unroll
     for (int r = 0; r < NZ; ++r) {
         const int rr = nR + r;
+        const LsConst& lc = D->ls[rr];
+        if (lc.kind != LS_GENERIC) {
+            ls_event_cached(S->P, kin, lc, e);
+            const C64 v = ls_value(lc, kin, e);
+            L[r] = (lc.kind == LS_BW) ? v : L[r] * v.re;
+            continue;
+        }
         const double m0 = D->m0[rr];
         if (S->type[rr] == PWA_BREITWIGNER)
             L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "pwa_amp.h"
+#include "pwa_lineshape.hpp"
 #include "pwa_math.hpp"
 
 namespace pwa {
@@ -71,6 +72,7 @@ struct DevStatic {
 struct DevDyn {
     double m0[PWA_MAX_RES];
     double g0[PWA_MAX_RES];
+    LsConst ls[PWA_MAX_RES];  // factorised lineshape constants (kind LS_GENERIC: direct evaluation)
     double km[PWA_KM_NCLASS][2 * PWA_KM_NPOLE * PWA_KM_NCH]; // g[pole][ch] then f[ch][ch]
     // K-matrix poles of each class (members with m0 > 0, in list order):
     // resonance index and threshold-shift channel (the width slot)
@@ -107,11 +109,20 @@ struct StageArgs {
 
 hipError_t launch_prep(const double* raw, int64_t ld_raw, int64_t n, double* ev, int64_t ld, hipStream_t st);
 hipError_t run_stages(int nz, const StageArgs& a, hipStream_t st);
+template <int NZ>
+hipError_t run_stages_nz(const StageArgs& a, hipStream_t st);  // one instantiation per NZ = 0..8
 hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, double* out, hipStream_t st);
 hipError_t launch_grad_reduce(const double* gpart, const int32_t* beg, int NG, double* out, hipStream_t st);
 size_t grad_lds_bytes(int NG);
 int inter_planes(int nz);
 constexpr int kBlock = 256;
+
+// Intermediate planes per event (see pwa_stages.hpp): quadratic-form
+// coefficients, angular amplitudes / adjoint seeds, Pc upper-triangle index.
+__host__ __device__ constexpr int n_pcs(int nz) { return 2 + 2 * nz + nz * nz; }
+__host__ __device__ constexpr int n_amp(int nz) { return 16 + 16 * nz; }
+__host__ __device__ constexpr int mo_index(int q, int r) { return r * (r - 1) / 2 + q; }  // q < r
+
 constexpr int kGradNClassParams = 36;  // per K-matrix class: g[16], f[16], pole masses[4]
 
 } // namespace pwa

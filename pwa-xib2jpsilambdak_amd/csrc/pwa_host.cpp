@@ -170,11 +170,14 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
             used[ik] = true;
         }
     }
+    // Lambda* grouped by spin, ordered by orbital momentum inside a group so
+    // consecutive resonances share the per-event lineshape factors
     int q = 0;
     for (int g = 0; g < 5; ++g) {
         S.lam_start[g] = q;
-        for (int r = 0; r < S.nR; ++r)
-            if (d.spin2[r] == 2 * g + 1) S.lam_idx[q++] = r;
+        for (int L = 0; L <= 8; ++L)
+            for (int r = 0; r < S.nR; ++r)
+                if (d.spin2[r] == 2 * g + 1 && S.L[r] == L) S.lam_idx[q++] = r;
     }
     S.lam_start[5] = q;
     S.km_nclass = 0;
@@ -210,6 +213,8 @@ void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         const int size = param_size(d, r);
         D.m0[r] = ires[size - 5];
         D.g0[r] = ires[size - 4];
+        const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
+        D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
         if (d.type[r] != PWA_KMATRIX) continue;
         const int ik = c.hs.ik[r];
         if (D.m0[r] > 0) {

@@ -1,0 +1,179 @@
+// Factorised lineshapes.  For the usual parameter ranges every lineshape of
+// the model is (lineshape x Lambda_b-vertex barrier term)
+//
+//   L(m) = C * G(m) * 1 / (a - i b),   a = m0^2 - m^2,  b = K * E(m)      (Breit-Wigner)
+//   L(m) = C * G(m)                                                        (vertex only)
+//
+// with per-resonance constants C, K, m0^2 (functions of m0, Gamma0 only,
+// computed once per evaluation on the host together with their m0 / Gamma0
+// derivatives) and per-event factors shared by every resonance of a chain
+// with the same orbital momenta (LR, LB):
+//
+//   G(m) = pR^LR / sqrt(B_LR(z)) * pB^LB / sqrt(B_LB(zB))
+//   E(m) = pR^(2 LR + 1) / (m B_LR(z))
+//
+// pR = breakup momentum of the resonance, pB = that of the Lambda_b vertex,
+// z = (ffr pR)^2, zB = (ffb pB)^2, B_L the Blatt-Weisskopf polynomials.  This
+// is the product of bw_lambda / bw_twobody and vertex_* (pwa_math.hpp) with the
+// m0-dependent factors collected into C and K; the resonances whose branch
+// conditions leave the usual ranges (below-threshold masses, the Sigma pi
+// width, negative masses) keep the direct evaluation (kind LS_GENERIC).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pwa_amp.h"
+#include "pwa_math.hpp"
+
+namespace pwa {
+
+enum LsKind { LS_GENERIC = 0, LS_BW = 1, LS_VERTEX = 2 };
+enum Chain { CH_LAMBDA = 0, CH_PC = 1, CH_ZCS = 2 };
+
+struct LsConst {
+    int32_t kind, LR, LB, pad;
+    double m2[3];  // m0^2 and its (m0, Gamma0) derivatives
+    double K[3];
+    double C[3];
+};
+
+// Orbital momentum of the Lambda_b vertex of each chain.
+PWA_HD int vertex_LB(int chain, int spin2) {
+    if (chain == CH_LAMBDA) return (spin2 <= 2) ? 0 : (spin2 - 3) / 2;
+    if (chain == CH_PC) return (spin2 - 1) / 2;
+    return spin2 >= 1 ? (spin2 - 1) / 2 : 0;
+}
+
+// Per-event kinematics of a chain at resonance mass m.
+struct ChainKin {
+    double s, imp, pR, pB;
+};
+PWA_HD ChainKin chain_kin(const Masses& P, int chain, double m) {
+    ChainKin k;
+    k.s = m * m;
+    k.imp = 1.0 / m;
+    if (chain == CH_LAMBDA) {
+        k.pR = breakup_d(m, P.m_p, P.m_k);
+        k.pB = breakup_d(P.m_lb, P.m_jpsi, m);
+    } else if (chain == CH_PC) {
+        k.pR = breakup_d(m, P.m_p, P.m_jpsi);
+        k.pB = breakup_d(P.m_lb, P.m_k, m);
+    } else {
+        k.pR = breakup_d(m, P.m_k, P.m_jpsi);
+        k.pB = breakup_d(P.m_lb, P.m_p, m);
+    }
+    return k;
+}
+
+PWA_HD double bpoly(double z, int L) { return (L >= 1 && L <= 5) ? bw_poly<double>(z, L) : 1.0; }
+
+// Event factors for orbital momenta (LR, LB): G and E as above, and the
+// vertex-only factor GB = pB^LB / sqrt(B_LB(zB)).
+struct LsEv {
+    int key;
+    double G, E, GB;
+};
+PWA_HD void ls_event(const Masses& P, const ChainKin& k, int LR, int LB, bool vertex_only, LsEv& e) {
+    const double rp = P.ffb * k.pB;
+    const double bzB = bpoly(rp * rp, LB);
+    const double pBL = ipow(k.pB, LB);
+    if (vertex_only) {
+        e.GB = pBL * sqrt(1.0 / bzB);
+        e.G = 0.0;
+        e.E = 0.0;
+    } else {
+        const double rr = P.ffr * k.pR;
+        const double bz = bpoly(rr * rr, LR);
+        const double X = 1.0 / (bz * bzB);
+        const double pL = ipow(k.pR, LR);
+        e.G = pL * pBL * sqrt(X);
+        e.E = pL * pL * k.pR * bzB * X * k.imp;
+        e.GB = 0.0;
+    }
+    e.key = vertex_only ? (64 + LB) : (LR * 8 + LB);
+}
+// Refresh the cached event factors when the orbital momenta change.
+PWA_HD void ls_event_cached(const Masses& P, const ChainKin& k, const LsConst& c, LsEv& e) {
+    const bool vo = c.kind == LS_VERTEX;
+    const int key = vo ? (64 + c.LB) : (c.LR * 8 + c.LB);
+    if (key != e.key) ls_event(P, k, c.LR, c.LB, vo, e);
+}
+
+// Value of a fast lineshape.
+PWA_HD C64 ls_value(const LsConst& c, const ChainKin& k, const LsEv& e) {
+    if (c.kind == LS_VERTEX) return C64(c.C[0] * e.GB, 0.0);
+    const double a = c.m2[0] - k.s;
+    const double b = c.K[0] * e.E;
+    const double t = c.C[0] * e.G / (a * a + b * b);
+    return C64(a * t, b * t);
+}
+
+// Value and (m0, Gamma0) derivatives of a fast lineshape.
+PWA_HD void ls_grad(const LsConst& c, const ChainKin& k, const LsEv& e, C64& v, C64& d0, C64& d1) {
+    if (c.kind == LS_VERTEX) {
+        v = C64(c.C[0] * e.GB, 0.0);
+        d0 = C64(c.C[1] * e.GB, 0.0);
+        d1 = C64(c.C[2] * e.GB, 0.0);
+        return;
+    }
+    const double a = c.m2[0] - k.s;
+    const double b = c.K[0] * e.E;
+    const double inv = 1.0 / (a * a + b * b);
+    const C64 w(a * inv, b * inv);  // 1 / (a - i b)
+    const C64 w2 = w * w;
+    const double cg = c.C[0] * e.G;
+    v = w * cg;
+    // d w = w^2 (d a - i d b) * (-1)
+    const C64 dw0 = w2 * C64(-c.m2[1], c.K[1] * e.E);
+    const C64 dw1 = w2 * C64(-c.m2[2], c.K[2] * e.E);
+    d0 = w * (c.C[1] * e.G) + dw0 * cg;
+    d1 = w * (c.C[2] * e.G) + dw1 * cg;
+}
+
+// Host: constants of resonance r for this evaluation (kind LS_GENERIC when
+// the branch conditions of the direct evaluation leave the usual ranges).
+// type: PWA_BREITWIGNER / PWA_NONRESONANT / PWA_KMATRIX (vertex term only).
+inline LsConst ls_const(const Masses& P, int chain, int type, int spin2, int LR, double m0v, double g0v) {
+    using D = Dual<2>;
+    LsConst c{};
+    c.LR = LR;
+    c.LB = vertex_LB(chain, spin2);
+    c.kind = LS_GENERIC;
+    const int LB = c.LB;
+    const D m0 = D::var(m0v, 0), g0 = D::var(g0v, 1);
+    // vertex constant (normalisation at the resonance mass for Lambda*, at m_lb otherwise)
+    double mB;
+    double ma, mb;
+    if (chain == CH_LAMBDA) { mB = P.m_jpsi; ma = P.m_p; mb = P.m_k; }
+    else if (chain == CH_PC) { mB = P.m_k; ma = P.m_p; mb = P.m_jpsi; }
+    else { mB = P.m_p; ma = P.m_k; mb = P.m_jpsi; }
+    const bool vstd = (m0v > 1e-10 && m0v + mB < P.m_lb);
+    if (LB > 0 && !vstd) return c;
+    D cv(1.0);
+    if (LB > 0) {
+        const D pB0 = breakup_dm(P.m_lb, mB, m0);
+        const D z0 = (P.ffb * pB0) * (P.ffb * pB0);
+        const D b0 = (LB >= 1 && LB <= 5) ? bw_poly(z0, LB) : D(1.0);
+        if (chain == CH_LAMBDA) cv = dsqrt(b0) / ipow(pB0, LB);
+        else cv = dsqrt(b0) * (1.0 / ipow(P.m_lb, LB));
+    }
+    D C, K, m2;
+    if (type == PWA_BREITWIGNER) {
+        if (!(m0v > ma + mb)) return c;
+        const D pR0 = breakup(m0, ma, mb);
+        const D z0 = (P.ffr * pR0) * (P.ffr * pR0);
+        const D b0 = (LR >= 1 && LR <= 5) ? bw_poly(z0, LR) : D(1.0);
+        C = dsqrt(b0) / ipow(pR0, LR) * bw_normalisation(m0, g0) * cv;
+        K = g0 * m0 * m0 * b0 / ipow(pR0, 2 * LR + 1);
+        m2 = m0 * m0;
+        c.kind = LS_BW;
+    } else {
+        C = cv;
+        c.kind = LS_VERTEX;
+    }
+    c.C[0] = C.v; c.C[1] = C.d[0]; c.C[2] = C.d[1];
+    c.K[0] = K.v; c.K[1] = K.d[0]; c.K[2] = K.d[1];
+    c.m2[0] = m2.v; c.m2[1] = m2.d[0]; c.m2[2] = m2.d[1];
+    return c;
+}
+
+} // namespace pwa

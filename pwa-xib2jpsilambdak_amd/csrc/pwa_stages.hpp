@@ -1,0 +1,407 @@
+// Per-event stage kernels of the likelihood (k_amp, k_shape, k_adj) and the
+// per-segment launcher, templated on the number of Pc-chain resonances NZ.
+// Each NZ is instantiated in its own translation unit (pwa_stages_nz.hip
+// compiled with -DPWA_NZ=k) so the kernels build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pwa_adjoint.hpp"
+#include "pwa_chains.hpp"
+
+namespace pwa {
+
+// ---------------------------------------------------- intermediate planes
+// pcs (quadratic form):  [H02, HNI, p0 (2 NZ), Md (NZ), Mo (NZ(NZ-1))]  -> 2 + 2 NZ + NZ^2
+// amp (gradient only):   k_amp writes [H0 (16), S_r (16 NZ)]; k_shape overwrites
+//                        them in place with the adjoint seeds [Y (16), U_r (16 NZ)]
+// aw  (gradient only):   the event's likelihood weight a (w/f for data, w for MC)
+
+// ------------------------------------------------------------ k_amp
+template <int NZ, bool GRAD>
+__global__ void __launch_bounds__(kBlock) k_amp(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                               const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                               const C64* __restrict__ cpl, double* __restrict__ pcs,
+                                               double* __restrict__ amp) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    auto P = [&](int p) -> double& { return pcs[(int64_t)p * ld + i]; };
+    C64 H0[8];
+    lambda_chain(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MKP), H0);
+    double hni = 0.0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) hni += norm2(H0[h]);
+    if (S->nX > 0) {
+        C64 HX[8];
+        zcs_chain(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MJPSIK), HX);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            hni += norm2(HX[h]);
+            H0[h] = H0[h] + HX[h];
+        }
+    }
+    double h02 = 0.0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) h02 += norm2(H0[h]);
+    P(0) = h02;
+    P(1) = hni;
+    if constexpr (GRAD) {
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            amp[(int64_t)(2 * h) * ld + i] = H0[h].re;
+            amp[(int64_t)(2 * h + 1) * ld + i] = H0[h].im;
+        }
+    }
+    if constexpr (NZ > 0) {
+        const ChainAng a = load_chain(ev, ld, i, E_Z_CB);
+        const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
+        double cp[8], sp[8];
+        powers(ld_d(ev, ld, i, E_Z_CZ), cp, 7);
+        powers(ld_d(ev, ld, i, E_Z_SZ), sp, 7);
+        C64 Sr[NZ][8];
+#pragma unroll
+        for (int r = 0; r < NZ; ++r) {
+            pc_angular(S, cpl, S->nR + r, a, cp, sp, cP, sP, Sr[r]);
+            C64 p0;
+            double md = 0.0;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                cmac(p0, conj(H0[h]), Sr[r][h]);
+                md += norm2(Sr[r][h]);
+            }
+            P(2 + 2 * r) = p0.re;
+            P(3 + 2 * r) = p0.im;
+            P(2 + 2 * NZ + r) = md;
+#pragma unroll
+            for (int q = 0; q < r; ++q) {
+                C64 m;
+#pragma unroll
+                for (int h = 0; h < 8; ++h) cmac(m, conj(Sr[q][h]), Sr[r][h]);
+                const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
+                P(o) = m.re;
+                P(o + 1) = m.im;
+            }
+            if constexpr (GRAD) {
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    amp[(int64_t)(16 + 16 * r + 2 * h) * ld + i] = Sr[r][h].re;
+                    amp[(int64_t)(16 + 16 * r + 2 * h + 1) * ld + i] = Sr[r][h].im;
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------- k_shape
+// Pc lineshapes over the event's sub-bins and the per-event likelihood term.
+template <int NZ, bool MC, bool GRAD, bool KM>
+__global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev, int64_t ld, int64_t n, int scheme,
+                                                 const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                 const double* __restrict__ pcs, double* __restrict__ amp,
+                                                 double* __restrict__ awp, double* __restrict__ fx_out,
+                                                 Partial* __restrict__ part, int noint, GradLayout G, int NGp,
+                                                 double* __restrict__ gpart) {
+    extern __shared__ double lds[];
+    Emitter em;
+    if constexpr (GRAD) em = Emitter::make(lds, NGp);
+    double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+    const int nb = S->scheme_nbins[scheme];
+    const double inb = 1.0 / nb;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        const int64_t gi = base + threadIdx.x;
+        const bool valid = gi < n;
+        const int64_t i = valid ? gi : n - 1;
+        auto P = [&](int p) { return pcs[(int64_t)p * ld + i]; };
+        const double w = ld_d(ev, ld, i, E_W);
+        const double mjp = ld_d(ev, ld, i, E_MJPSIP);
+        double f = P(0), fni = P(1);
+        C64 p0[NZ > 0 ? NZ : 1], Mo[NZ > 0 ? NZ : 1][NZ > 0 ? NZ : 1];
+        double Md[NZ > 0 ? NZ : 1];
+        // lineshape-parameter sums (unweighted) for the gradient
+        double gm[NZ > 0 ? NZ : 1], gw[NZ > 0 ? NZ : 1];
+        C64 BW1[NZ > 0 ? NZ : 1], B2o[NZ > 0 ? NZ : 1][NZ > 0 ? NZ : 1];
+        double B2d[NZ > 0 ? NZ : 1];
+        double kgb[16], kfb[16], kmb[4];
+        // the gradient is carried for at most one K-matrix class (the host
+        // rejects models with more for pwa_eval_grad); the forward handles all
+        const int ikc = (GRAD && S->km_nclass > 0) ? S->km_class[0] : -1;
+        if constexpr (NZ > 0) {
+#pragma unroll
+            for (int r = 0; r < NZ; ++r) {
+                p0[r] = C64(P(2 + 2 * r), P(3 + 2 * r));
+                Md[r] = P(2 + 2 * NZ + r);
+                gm[r] = 0.0;
+                gw[r] = 0.0;
+                BW1[r] = C64(0.0, 0.0);
+                B2d[r] = 0.0;
+#pragma unroll
+                for (int q = 0; q < r; ++q) {
+                    const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
+                    Mo[q][r] = C64(P(o), P(o + 1));
+                    B2o[q][r] = C64(0.0, 0.0);
+                }
+            }
+            if constexpr (GRAD) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    kgb[q] = 0.0;
+                    kfb[q] = 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) kmb[q] = 0.0;
+            }
+            const double sig = S->mres[0] + mjp * (S->mres[1] + mjp * S->mres[2]);
+            double acc = 0.0;
+            C64 L[NZ];
+            for (int b = 0; b < nb; ++b) {
+                const double mb = mjp + S->scheme_z[scheme][b] * sig;
+                if constexpr (!GRAD) {
+                    pc_lineshapes<NZ, KM>(S, D, mb, L);
+                    acc += pc_form<NZ>(L, p0, Md, Mo);
+                } else {
+                    C64 Mt[NZ], dL0[NZ], dL1[NZ];
+                    double Vx[NZ];
+                    KMState st;
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) Mt[r] = C64(1.0, 0.0);
+                    if constexpr (KM)
+                    for (int c = 0; c < S->km_nclass; ++c) {
+                        const int ik = S->km_class[c];
+                        C64 Mtr[8];
+                        if (ik == ikc) kmatrix_state(S, D, ik, mb, st, Mtr);
+                        else kmatrix(S, D, ik, mb, Mtr);
+#pragma unroll
+                        for (int r = 0; r < NZ; ++r)
+                            if (S->ik[S->nR + r] == ik) Mt[r] = select8(Mtr, S->ikm[S->nR + r]);
+                    }
+                    const ChainKin kin = chain_kin(S->P, CH_PC, mb);
+                    LsEv e;
+                    e.key = -1;
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) pc_ls_grad(S, D, S->nR + r, kin, e, mb, Mt[r], L[r], dL0[r], dL1[r], Vx[r]);
+                    acc += pc_form<NZ>(L, p0, Md, Mo);
+                    C64 Ab[8];
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) {
+                        BW1[r] = BW1[r] + L[r];
+                        B2d[r] += norm2(L[r]);
+#pragma unroll
+                        for (int q = 0; q < r; ++q) B2o[q][r] = B2o[q][r] + L[q] * conj(L[r]);
+                        // P_r = p0_r + sum_q conj(L_q) M[q][r]
+                        C64 Pr = p0[r];
+                        rmac(Pr, Md[r], conj(L[r]));
+#pragma unroll
+                        for (int q = 0; q < NZ; ++q) {
+                            if (q < r) Pr = Pr + conj(L[q]) * Mo[q][r];
+                            else if (q > r) Pr = Pr + conj(L[q]) * conj(Mo[r][q]);
+                        }
+                        gm[r] += re_mul(Pr, dL0[r]);
+                        gw[r] += re_mul(Pr, dL1[r]);
+                        if (ikc >= 0 && S->ik[S->nR + r] == ikc) {
+                            const C64 add = Pr * Vx[r];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q)
+                                if (q == S->ikm[S->nR + r]) Ab[q] = Ab[q] + add;
+                        }
+                    }
+                    if constexpr (KM)
+                        if (ikc >= 0) kmatrix_adjoint(D, ikc, st, Ab, kgb, kfb, kmb);
+                }
+            }
+            f += acc * inb;
+            if (MC && noint) {
+                if (scheme != 0) pc_lineshapes<NZ, KM>(S, D, mjp, L);
+                C64 zero[NZ];
+                fni += pc_form<NZ>(L, zero, Md, Mo);
+            }
+        }
+        double v0, v1, aw;
+        if (MC) {
+            const bool unw = (w <= -1000.0);
+            v0 = unw ? f : f * w;
+            v1 = (noint && !unw) ? fni * w : 0.0;
+            aw = unw ? 1.0 : w;
+            if (valid && fx_out) fx_out[gi] = v0;
+        } else {
+            if (f > 0) v0 = log(f) * w;
+            else v0 = (S->flags & PWA_F_NEG_FX_PENALTY) ? -1e12 : 0.0;
+            v1 = (f > 0) ? 0.0 : 1.0;
+            aw = (f > 0) ? w / f : 0.0;
+            if (valid && fx_out) fx_out[gi] = f;
+        }
+        if (!valid) {
+            v0 = 0.0;
+            v1 = 0.0;
+            aw = 0.0;
+        }
+        kadd(s0, c0, v0);
+        kadd(s1, c1, v1);
+        if constexpr (GRAD) {
+            if constexpr (NZ > 0) {
+                // adjoint seeds, in place of H0 and S_r:
+                //   Y_h   = H0_h + sum_r BW1_r S_rh
+                //   U_rh  = BW1_r conj(H0_h) + sum_q BW2_rq conj(S_qh)
+                // with BW1_r = <L_r>, BW2_rq = <L_r conj(L_q)> over the sub-bins
+                if (valid) {
+                    C64 b1[NZ];
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) b1[r] = BW1[r] * inb;
+#pragma unroll 1
+                    for (int h = 0; h < 8; ++h) {
+                        auto A = [&](int p) -> double& { return amp[(int64_t)p * ld + i]; };
+                        const C64 h0(A(2 * h), A(2 * h + 1));
+                        C64 sq[NZ];
+#pragma unroll
+                        for (int q = 0; q < NZ; ++q) sq[q] = C64(A(16 + 16 * q + 2 * h), A(17 + 16 * q + 2 * h));
+                        C64 y = h0;
+#pragma unroll
+                        for (int r = 0; r < NZ; ++r) cmac(y, b1[r], sq[r]);
+                        A(2 * h) = y.re;
+                        A(2 * h + 1) = y.im;
+#pragma unroll
+                        for (int r = 0; r < NZ; ++r) {
+                            C64 u = b1[r] * conj(h0);
+#pragma unroll
+                            for (int q = 0; q < NZ; ++q) {
+                                C64 b2;
+                                if (q == r) b2 = C64(B2d[r], 0.0);
+                                else if (q < r) b2 = conj(B2o[q][r]);
+                                else b2 = B2o[r][q];
+                                cmac(u, b2 * inb, conj(sq[q]));
+                            }
+                            A(16 + 16 * r + 2 * h) = u.re;
+                            A(17 + 16 * r + 2 * h) = u.im;
+                        }
+                    }
+                }
+                const double sc = 2.0 * aw * inb;
+                em.begin(2 * NZ);
+#pragma unroll
+                for (int r = 0; r < NZ; ++r) {
+                    em.put(2 * r, sc * gm[r]);
+                    em.put(2 * r + 1, sc * gw[r]);
+                }
+                em.commit(2 * NZ, G.res(S->nR));
+                if (KM && ikc >= 0) {
+                    const int o = G.km(ikc);
+                    em.begin(16);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) em.put(q, sc * kgb[q]);
+                    em.commit(16, o);
+                    em.begin(16);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) em.put(q, sc * kfb[q]);
+                    em.commit(16, o + 16);
+                    em.begin(4);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) em.put(q, sc * kmb[q]);
+                    em.commit(4, o + 32);
+                }
+            }
+            if (valid) awp[i] = aw;
+        }
+    }
+    if constexpr (GRAD) {
+        if (em.cnt > 0) em.flush();
+    }
+    block_reduce2(s0, c0, s1, c1);
+    if (threadIdx.x == 0) part[blockIdx.x] = Partial{s0, c0, s1, c1};
+    if constexpr (GRAD) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+    }
+}
+
+// ------------------------------------------------------------- k_adj
+// Coupling adjoint from the seeds k_shape leaves in the amp planes: Y for the
+// Lambda* / Zcs couplings and lineshapes, U_r for the Pc couplings; every
+// contribution weighted by 2 a.
+template <int NZ>
+__global__ void __launch_bounds__(kBlock) k_adj(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                               const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                               const C64* __restrict__ cpl, const double* __restrict__ amp,
+                                               const double* __restrict__ awp, GradLayout G, int NGp,
+                                               double* __restrict__ gpart) {
+    extern __shared__ double lds[];
+    Emitter em = Emitter::make(lds, NGp);
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        const int64_t gi = base + threadIdx.x;
+        const bool valid = gi < n;
+        const int64_t i = valid ? gi : n - 1;
+        auto A = [&](int p) { return amp[(int64_t)p * ld + i]; };
+        const double a2 = valid ? 2.0 * awp[i] : 0.0;
+        {
+            C64 Y[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) Y[h] = C64(A(2 * h), A(2 * h + 1));
+            lambda_adjoint(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MKP), Y, a2, G, em);
+            if (S->nX > 0) zcs_adjoint(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MJPSIK), Y, a2, G, em);
+        }
+        if constexpr (NZ > 0) {
+            const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
+            const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
+            double cp[8], sp[8];
+            powers(ld_d(ev, ld, i, E_Z_CZ), cp, 7);
+            powers(ld_d(ev, ld, i, E_Z_SZ), sp, 7);
+#pragma unroll 1
+            for (int r = 0; r < NZ; ++r) {
+                C64 U[8];
+#pragma unroll
+                for (int h = 0; h < 8; ++h) U[h] = C64(A(16 + 16 * r + 2 * h), A(17 + 16 * r + 2 * h));
+                pc_coupling_adjoint(S, S->nR + r, za, cp, sp, cP, sP, U, a2, G, em);
+            }
+        }
+    }
+    if (em.cnt > 0) em.flush();
+    __syncthreads();
+    for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+}
+
+
+template <int NZ>
+hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
+    const unsigned eblocks = (unsigned)((a.n + kBlock - 1) / kBlock);
+    double* pcs = a.inter;
+    double* amp = a.inter + (int64_t)n_pcs(NZ) * a.ld;
+    double* awp = amp + (int64_t)n_amp(NZ) * a.ld;
+    GradLayout G{a.ncpl, a.nres, a.ng};
+    const int NGp = (a.ng + 1) & ~1;
+    const size_t lds = a.grad ? grad_lds_bytes(a.ng) : 0;
+    hipError_t e;
+    if (a.grad) hipLaunchKernelGGL((k_amp<NZ, true>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.cpl, pcs, amp);
+    else hipLaunchKernelGGL((k_amp<NZ, false>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.cpl, pcs, amp);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned sblocks = (unsigned)a.shape_blocks;
+#define PWA_SHAPE(MC, GR, KM)                                                                                      \
+    do {                                                                                                           \
+        if (GR) {                                                                                                  \
+            e = hipFuncSetAttribute((const void*)k_shape<NZ, MC, GR, KM>,                                          \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                         \
+            if (e != hipSuccess) return e;                                                                         \
+        }                                                                                                          \
+        hipLaunchKernelGGL((k_shape<NZ, MC, GR, KM>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,        \
+                           a.scheme, a.S, a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart_shape);            \
+    } while (0)
+    if (a.km) {
+        if (a.mc && a.grad) PWA_SHAPE(true, true, true);
+        else if (a.mc) PWA_SHAPE(true, false, true);
+        else if (a.grad) PWA_SHAPE(false, true, true);
+        else PWA_SHAPE(false, false, true);
+    } else {
+        if (a.mc && a.grad) PWA_SHAPE(true, true, false);
+        else if (a.mc) PWA_SHAPE(true, false, false);
+        else if (a.grad) PWA_SHAPE(false, true, false);
+        else PWA_SHAPE(false, false, false);
+    }
+#undef PWA_SHAPE
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.grad) {
+        e = hipFuncSetAttribute((const void*)k_adj<NZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_adj<NZ>), dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n, a.S,
+                           a.D, a.cpl, amp, awp, G, NGp, a.gpart_adj);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+} // namespace pwa

@@ -1,0 +1,11 @@
+// One instantiation of the per-segment stage pipeline (pwa_stages.hpp) for
+// NZ = PWA_NZ Pc-chain resonances; compiled once per NZ = 0..8.
+#include "pwa_stages.hpp"
+
+#ifndef PWA_NZ
+#error "compile with -DPWA_NZ=<number of Pc-chain resonances>"
+#endif
+
+namespace pwa {
+template hipError_t run_stages_nz<PWA_NZ>(const StageArgs& a, hipStream_t st);
+} // namespace pwa
