@@ -34,22 +34,69 @@ PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
 PEAK_HBM = 8000.0  # GB/s
 
 
-def algorithmic_flops(model, n_events, nbins, smeared, grad=True):
-    """Work model of the hot path per event: coherent contraction
-    F_fwd = 8 H K (H = 8 helicity rows, K = complex coupling columns),
-    adjoint F_adj = 2 F_fwd, smearing 8 S R^2 + 8 H R^2 for smeared events,
-    K-matrix row-0 solve 299 S n_cls per smeared event."""
+def coupling_counts(d):
+    """Complex helicity couplings of the Lambda* + Zcs chains and of the Pc
+    chain (the enumeration of the amplitude kernels)."""
+    k_lx = k_pc = 0
+    for r in range(d.n_lambda + d.n_pc + d.n_zcs):
+        j2 = d.spin2[r]
+        if r < d.n_lambda:
+            k_lx += 2 * (4 if j2 == 1 else 6)
+        elif r < d.n_lambda + d.n_pc:
+            k_pc += sum(1 for k in range(2) for ps in range(3) for iz in range(2) for ip in range(2)
+                        if abs(2 * (ps - 1) - (2 * ip - 1)) <= j2)
+        else:
+            k_lx += sum(1 for k in range(2) for ps in range(3) if 2 * abs(ps - 1) <= j2
+                        for ip in range(2) for iz in range(2) if abs((2 * ip - 1) + (2 * iz - 1)) <= j2)
+    return k_lx, k_pc
+
+
+STAGES = ("lam", "pcang", "shape", "adj_lam", "adj_pc")
+
+
+def work_model(model):
+    """Per-event algorithmic work of each stage kernel (FP64 flops, complex
+    MAC = 8, complex mul = 6; transcendentals not counted).  The coherent
+    contraction F_fwd = 8 H K = 64 K and its adjoint F_adj = 2 F_fwd are split
+    by coupling class between the kernels that do them:
+      k_lam     64 K_lx                      (Lambda* + Zcs contraction)
+      k_pcang   64 K_pc + 8 H R^2            (Pc contraction, quadratic-form Gram terms)
+      k_shape   8 S R^2 + 299 S n_cls        (smeared events: sub-bin quadratic
+                                              forms and K-matrix row-0 solves)
+      k_adj_lam 128 K_lx,  k_adj_pc 128 K_pc
+    K = complex helicity couplings, H = 8 helicity rows, R = Pc-chain
+    resonances, S = sub-bins, n_cls = K-matrix J^P classes."""
     d = model.desc
     from pwa_amd._lib import helicity_couplings
-    K = len(helicity_couplings(d, model.params)) / 2.0
+    K = len(helicity_couplings(d, model.params))  # complex couplings
+    k_lx, k_pc = coupling_counts(d)
+    assert k_lx + k_pc == K, (k_lx, k_pc, K)
     R = d.n_pc
     ncls = len({(d.spin2[r], d.parity[r]) for r in range(d.n_lambda, d.n_lambda + d.n_pc)
                 if d.type[r] == abi.PWA_KMATRIX})
-    f = 64.0 * K * (3.0 if grad else 1.0)
-    S = nbins if smeared else 1
-    f += 8.0 * S * R * R + 8.0 * 8 * R * R
-    f += 299.0 * S * ncls
-    return f * n_events
+
+    def flops(stage, n_events, nbins, smeared):
+        per = {"lam": 64.0 * k_lx, "pcang": 64.0 * k_pc + 64.0 * R * R,
+               "shape": (8.0 * nbins * R * R + 299.0 * nbins * ncls) if smeared else 0.0,
+               "adj_lam": 128.0 * k_lx, "adj_pc": 128.0 * k_pc}[stage]
+        return per * n_events
+
+    return K, flops
+
+
+def traffic_per_launch(stage, n_events, launches_per_step):
+    """HBM bytes per launch of a stage kernel from the per-event figure that
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded (profiles/hbm_traffic.json,
+    already scaled by the copy-kernel calibration); None without it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    per_event = t.get("bytes_per_event", {}).get("k_" + stage)
+    if per_event is None:
+        return None
+    return round(per_event * n_events / max(launches_per_step, 1))
 
 
 def build_workload(nd, nm, seed=2024):
@@ -150,28 +197,33 @@ def main():
     print(f"[bench] forward: {t_fwd * 1e3:.3f} ms/eval -> FD gradient {1 / fd_rate:.3f} s", file=sys.stderr,
           flush=True)
 
-    # --- roofline of the dominant kernel (the MC-sample gradient kernel,
-    #     all its launches of one evaluation), device times from HIP events
-    frac_smeared = float(np.mean(ms > 0))
-    mc_launch_ms, mc_flops, mc_events = 0.0, 0.0, 0
+    # --- roofline of the dominant stage kernel: its device time summed over
+    #     all its launches of the timed evaluations (HIP events on the
+    #     context's stream, recorded around each kernel)
+    K, flops = work_model(model)
     nb = model.desc.scheme_nbins[0]
+    tot = {st: [0.0, 0.0, 0] for st in STAGES}  # ms, flops, launches
     for step in launches:
         for L in step:
-            if L["grad"] and L["kind"] == abi.PWA_MC:
-                mc_launch_ms += L["ms"]
-                mc_events += L["n"]
-                mc_flops += algorithmic_flops(model, L["n"], nb, L["scheme"] > 0)
-    n_launch = sum(1 for step in launches for L in step if L["grad"] and L["kind"] == abi.PWA_MC)
-    avg_ms = mc_launch_ms / n_launch
-    avg_flops = mc_flops / n_launch
-    achieved = avg_flops / (avg_ms * 1e-3) / 1e12
-    hbm_bytes = 368.0  # bytes read per event (46 FP64 planes)
+            for st in tot:
+                t_ms = L["ms_" + st]
+                if t_ms <= 0.0:
+                    continue
+                tot[st][0] += t_ms
+                tot[st][1] += flops(st, L["n"], nb, L["scheme"] > 0)
+                tot[st][2] += 1
+    dom = max(tot, key=lambda st: tot[st][0])
+    ms_sum, fl_sum, nlaunch = tot[dom]
+    avg_ms = ms_sum / nlaunch
+    achieved = fl_sum / (ms_sum * 1e-3) / 1e12
+    share = {st: round(v[0] / args.steps, 4) for st, v in tot.items()}
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
-            "kernel": "k_grad<3,MC> (per launch; MC segments smeared + unsmeared)",
-            "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": avg_flops,
-            "hbm_input_GBps": round(hbm_bytes * (mc_events / n_launch) / (avg_ms * 1e-3) / 1e9, 1)}
+            "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic_per_launch(dom, nd + nm, nlaunch // args.steps),
+            "kernel": f"k_{dom} (gradient pipeline; all data + MC segment launches)",
+            "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": fl_sum / nlaunch,
+            "stage_ms_per_step": share, "K_couplings": K}
 
+    frac_smeared = float(np.mean(ms > 0))
     out = {
         "metric": "NLL+grad evals/s (300k ev, 3M MC); wall-time to global min vs plain GPU FCN",
         "value": round(1.0 / t_grad, 3),

@@ -232,9 +232,14 @@ int pwa_amp2(pwa_ctx* ctx, int kind, const double* params, double* amp2);
    context's stream): all kernels, and the longest single kernel launch.   */
 int pwa_last_timing(pwa_ctx* ctx, double* ms_total, double* ms_main);
 
-/* Per-launch device times of the last evaluation: for launch q, ms[q], the
-   sample kind, smearing scheme, whether it was the gradient kernel, and its
-   event count.  *n receives the number of launches (up to cap are written). */
+/* Per-segment device times of the last evaluation (HIP events on the
+   context's stream).  Segment q is one run of the stage kernels over the
+   events of one sample and smearing scheme; ms[5q .. 5q+4] are the times of
+   its five kernels in launch order: Lambda* and Zcs amplitude (k_lam), Pc angular
+   sums (k_pcang), Pc lineshapes and likelihood term (k_shape), Lambda* and Zcs
+   adjoint (k_adj_lam), Pc adjoint (k_adj_pc); kernels that did not run read
+   0.  kind, scheme, is_grad and nevents describe the segment.  *n receives
+   the number of segments (up to cap are written; ms holds 5 cap values). */
 int pwa_last_launches(pwa_ctx* ctx, int32_t cap, int32_t* n, double* ms, int32_t* kind,
                       int32_t* scheme, int32_t* is_grad, int64_t* nevents);
 

@@ -95,37 +95,36 @@ __device__ __forceinline__ C64 cval(const Cx<D2>& x) { return C64(x.re.v, x.im.v
 __device__ __forceinline__ C64 cder(const Cx<D2>& x, int k) { return C64(x.re.d[k], x.im.d[k]); }
 
 // ------------------------------------- lineshape x vertex with derivatives
-// Value and (m0, Gamma0) derivatives of the Lambda* lineshape x vertex term.
-__device__ __forceinline__ void lambda_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                               const ChainKin& kin, LsEv& e, double mkp, C64& v, C64& d0,
-                                               C64& d1) {
+// Value and (m0, Gamma0) derivatives of a Lambda* or Zcs lineshape x vertex
+// term (factorised form, or the generic buffer).
+__device__ __forceinline__ void chain_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                              const ChainKin& kin, LsEv& e, const double* __restrict__ gen,
+                                              int64_t ld, int64_t i, C64& v, C64& d0, C64& d1) {
     const LsConst& lc = D->ls[r];
     if (lc.kind != LS_GENERIC) {
         ls_event_cached(S->P, kin, lc, e);
         ls_grad(lc, kin, e, v, d0, d1);
         return;
     }
-    const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
-    Cx<D2> Ls(D2(1.0), D2(0.0));
-    if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<D2>(S->P, m0, g0, mkp, S->L[r]);
-    Ls = Ls * vertex_lambda<D2>(S->P, S->spin2[r], mkp, m0);
-    v = cval(Ls);
-    d0 = cder(Ls, 0);
-    d1 = cder(Ls, 1);
+    const double* g = gen + (int64_t)(kGenPlanes * lc.slot) * ld + i;
+    v = C64(g[0], g[ld]);
+    d0 = C64(g[2 * ld], g[3 * ld]);
+    d1 = C64(g[4 * ld], g[5 * ld]);
 }
 
-// Same for a Zcs resonance.
-__device__ __forceinline__ void zcs_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                            const ChainKin& kin, LsEv& e, double mjk, C64& v, C64& d0, C64& d1) {
-    const LsConst& lc = D->ls[r];
-    if (lc.kind != LS_GENERIC) {
-        ls_event_cached(S->P, kin, lc, e);
-        ls_grad(lc, kin, e, v, d0, d1);
-        return;
-    }
+// Direct (generic) evaluation of Lambda* / Zcs resonance r with derivatives
+// (bw_lambda x vertex_lambda, bw_twobody x vertex_zcs).
+__device__ __forceinline__ void chain_ls_generic(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                                 double m, C64& v, C64& d0, C64& d1) {
     const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
-    Cx<D2> Ls = bw_twobody<D2>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
-    Ls = Ls * vertex_zcs<D2>(S->P, S->spin2[r], mjk, m0);
+    Cx<D2> Ls(D2(1.0), D2(0.0));
+    if (r < S->nR) {
+        if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<D2>(S->P, m0, g0, m, S->L[r]);
+        Ls = Ls * vertex_lambda<D2>(S->P, S->spin2[r], m, m0);
+    } else {
+        Ls = bw_twobody<D2>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, m, S->L[r]);
+        Ls = Ls * vertex_zcs<D2>(S->P, S->spin2[r], m, m0);
+    }
     v = cval(Ls);
     d0 = cder(Ls, 0);
     d1 = cder(Ls, 1);
@@ -168,8 +167,9 @@ __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, cons
 
 // ---------------------------------------------------------- Lambda* adjoint
 __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                               const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld, int64_t i,
-                               double mkp, const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
+                                               const C64* __restrict__ cpl, const double* __restrict__ ev,
+                                               const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
+                                               const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
     const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
     const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
     const double cL = ld_d(ev, ld, i, E_L_CL), sL = ld_d(ev, ld, i, E_L_SL);
@@ -194,9 +194,6 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                          conj(Y[4 * ii + 2 * jj + k]));
             Q[k][m] = ph[m] * acc;
         }
-    double cp[10], sp[10];
-    powers(cL, cp, 9);
-    powers(sL, sp, 9);
     const ChainKin kin = chain_kin(S->P, CH_LAMBDA, mkp);
     LsEv ev_ls;
     ev_ls.key = -1;
@@ -204,18 +201,12 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
         double dl[2][6];
-        switch (g) {
-        case 0: dl_table<1>(cp, sp, dl); break;
-        case 1: dl_table<3>(cp, sp, dl); break;
-        case 2: dl_table<5>(cp, sp, dl); break;
-        case 3: dl_table<7>(cp, sp, dl); break;
-        default: dl_table<9>(cp, sp, dl); break;
-        }
+        lambda_dl(g, cL, sL, dl);
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
             C64 Lv, dL0, dL1;
-            lambda_ls_grad(S, D, r, kin, ev_ls, mkp, Lv, dL0, dL1);
+            chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
             Lv = Lv * a2;
             const C64* c = cpl + S->ampcount[r];
             const int base = G.cpl(S->ampcount[r]);
@@ -267,14 +258,12 @@ __device__ __forceinline__ void chain_adjoint(const ChainAng& a, const C64 W[8],
 
 // ------------------------------------------------------------- Zcs adjoint
 __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                            const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld, int64_t i,
-                            double mjk, const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
+                                            const C64* __restrict__ cpl, const double* __restrict__ ev,
+                                            const double* __restrict__ gen, int64_t ld, int64_t i, double mjk,
+                                            const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
     const ChainAng a = load_chain(ev, ld, i, E_X_CB);
     const double cX = ld_d(ev, ld, i, E_X_CZ), sX = ld_d(ev, ld, i, E_X_SZ);
     const double cP = ld_d(ev, ld, i, E_X_CP), sP = ld_d(ev, ld, i, E_X_SP);
-    double cp[8], sp[8];
-    powers(cX, cp, 7);
-    powers(sX, sp, 7);
     C64 cY[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) cY[h] = conj(Y[h]);
@@ -287,14 +276,9 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
         const int r = S->nR + S->nZ + q;
         const int J2 = S->spin2[r];
         double dx[3][3];
-        switch (J2) {
-        case 0: dx_table<0>(cp, sp, dx); break;
-        case 2: dx_table<2>(cp, sp, dx); break;
-        case 4: dx_table<4>(cp, sp, dx); break;
-        default: dx_table<6>(cp, sp, dx); break;
-        }
+        zcs_dx(J2, cX, sX, dx);
         C64 Lv, dL0, dL1;
-        zcs_ls_grad(S, D, r, kin, ev_ls, mjk, Lv, dL0, dL1);
+        chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
         Lv = Lv * a2;
         const C64* c = cpl + S->ampcount[r];
         C64 R;
@@ -330,17 +314,12 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
 }
 
 // ---------------------------------------------------- Pc coupling adjoint
-__device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int r, const ChainAng& a, const double* cp,
-                                    const double* sp, double cP, double sP, const C64 U[8], double a2,
-                                    const GradLayout& G, Emitter& em) {
+__device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int r, const ChainAng& a, double cZ,
+                                                    double sZ, double cP, double sP, const C64 U[8], double a2,
+                                                    const GradLayout& G, Emitter& em) {
     const int J2 = S->spin2[r];
     double dz[2][4];
-    switch (J2) {
-    case 1: dz_table<1>(cp, sp, dz); break;
-    case 3: dz_table<3>(cp, sp, dz); break;
-    case 5: dz_table<5>(cp, sp, dz); break;
-    default: dz_table<7>(cp, sp, dz); break;
-    }
+    pc_dz(J2, cZ, sZ, dz);
     C64 T[2][3][2];
     chain_adjoint(a, U, T);
     int n = 0;

@@ -64,48 +64,60 @@ __device__ __forceinline__ C64 cpow_int(C64 e, int n) {
     return n > 0 ? e : conj(e);
 }
 
-// ------------------------------------------------ lineshape x vertex values
-// Lambda* resonance r at m(K p): factorised form when available, else the
-// direct evaluation (bw_lambda x vertex_lambda).
-__device__ __forceinline__ C64 lambda_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                         const ChainKin& kin, LsEv& e, double mkp) {
-    const LsConst& lc = D->ls[r];
-    if (lc.kind != LS_GENERIC) {
-        ls_event_cached(S->P, kin, lc, e);
-        return ls_value(lc, kin, e);
+// Lambda* Wigner table of spin group g (2J = 2g + 1) at theta_L / 2 with
+// cos = c, sin = s; the powers are built per group so only the needed ones
+// are live.
+__device__ __forceinline__ void lambda_dl(int g, double c, double s, double dl[2][6]) {
+    switch (g) {
+    case 0: { double cp[2], sp[2]; powers(c, cp, 1); powers(s, sp, 1); dl_table<1>(cp, sp, dl); } break;
+    case 1: { double cp[4], sp[4]; powers(c, cp, 3); powers(s, sp, 3); dl_table<3>(cp, sp, dl); } break;
+    case 2: { double cp[6], sp[6]; powers(c, cp, 5); powers(s, sp, 5); dl_table<5>(cp, sp, dl); } break;
+    case 3: { double cp[8], sp[8]; powers(c, cp, 7); powers(s, sp, 7); dl_table<7>(cp, sp, dl); } break;
+    default: { double cp[10], sp[10]; powers(c, cp, 9); powers(s, sp, 9); dl_table<9>(cp, sp, dl); } break;
     }
-    const double m0 = D->m0[r], g0 = D->g0[r];
-    C64 Ls(1.0, 0.0);
-    if (S->type[r] == PWA_BREITWIGNER) Ls = bw_lambda<double>(S->P, m0, g0, mkp, S->L[r]);
-    return Ls * vertex_lambda<double>(S->P, S->spin2[r], mkp, m0);
 }
 
-// Zcs resonance r at m(J/psi K).
-__device__ __forceinline__ C64 zcs_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                      const ChainKin& kin, LsEv& e, double mjk) {
+// Pc Wigner values d^J(theta_Z) for 2J = J2 from cos, sin of theta_Z / 2.
+__device__ __forceinline__ void pc_dz(int J2, double c, double s, double dz[2][4]) {
+    switch (J2) {
+    case 1: { double cp[2], sp[2]; powers(c, cp, 1); powers(s, sp, 1); dz_table<1>(cp, sp, dz); } break;
+    case 3: { double cp[4], sp[4]; powers(c, cp, 3); powers(s, sp, 3); dz_table<3>(cp, sp, dz); } break;
+    case 5: { double cp[6], sp[6]; powers(c, cp, 5); powers(s, sp, 5); dz_table<5>(cp, sp, dz); } break;
+    default: { double cp[8], sp[8]; powers(c, cp, 7); powers(s, sp, 7); dz_table<7>(cp, sp, dz); } break;
+    }
+}
+
+// Zcs Wigner values d^J(theta_X) for integer J (2J = J2).
+__device__ __forceinline__ void zcs_dx(int J2, double c, double s, double dx[3][3]) {
+    switch (J2) {
+    case 0: { double cp[1], sp[1]; powers(c, cp, 0); powers(s, sp, 0); dx_table<0>(cp, sp, dx); } break;
+    case 2: { double cp[3], sp[3]; powers(c, cp, 2); powers(s, sp, 2); dx_table<2>(cp, sp, dx); } break;
+    case 4: { double cp[5], sp[5]; powers(c, cp, 4); powers(s, sp, 4); dx_table<4>(cp, sp, dx); } break;
+    default: { double cp[7], sp[7]; powers(c, cp, 6); powers(s, sp, 6); dx_table<6>(cp, sp, dx); } break;
+    }
+}
+
+// ------------------------------------------------ lineshape x vertex values
+// Lambda* or Zcs resonance r (factorised form, or the generic buffer).
+__device__ __forceinline__ C64 chain_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
+                                        const ChainKin& kin, LsEv& e, const double* __restrict__ gen, int64_t ld,
+                                        int64_t i) {
     const LsConst& lc = D->ls[r];
     if (lc.kind != LS_GENERIC) {
         ls_event_cached(S->P, kin, lc, e);
         return ls_value(lc, kin, e);
     }
-    const double m0 = D->m0[r], g0 = D->g0[r];
-    const C64 Ls = bw_twobody<double>(S->P.m_k, S->P.m_jpsi, S->P.ffr, m0, g0, mjk, S->L[r]);
-    return Ls * vertex_zcs<double>(S->P, S->spin2[r], mjk, m0);
+    const double* g = gen + (int64_t)(kGenPlanes * lc.slot) * ld + i;
+    return C64(g[0], g[ld]);
 }
 
 // ------------------------------------------------------ Lambda* chain
 // H[4i+2j+k] += sum_r L_r sum_m c_{r,k,m} DL_J[k][m] A[i][j][m]   (GetAmpLz, src/kernel_calEva.cu:85-190)
 __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                              const C64* __restrict__ cpl, const double* __restrict__ ev,
-                                             int64_t ld, int64_t i, double mkp, C64 H[8]) {
-    const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
-    const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
+                                             const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
+                                             C64 H[8]) {
     const double cL = ld_d(ev, ld, i, E_L_CL), sL = ld_d(ev, ld, i, E_L_SL);
-    const C64 eK = ld_c(ev, ld, i, E_L_EKR, E_L_EKI);
-    const C64 eM = ld_c(ev, ld, i, E_L_EMR, E_L_EMI);
-    double cp[10], sp[10];
-    powers(cL, cp, 9);
-    powers(sL, sp, 9);
     const ChainKin kin = chain_kin(S->P, CH_LAMBDA, mkp);
     LsEv ev_ls;
     ev_ls.key = -1;
@@ -115,17 +127,11 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
         double dl[2][6];
-        switch (g) {
-        case 0: dl_table<1>(cp, sp, dl); break;
-        case 1: dl_table<3>(cp, sp, dl); break;
-        case 2: dl_table<5>(cp, sp, dl); break;
-        case 3: dl_table<7>(cp, sp, dl); break;
-        default: dl_table<9>(cp, sp, dl); break;
-        }
+        lambda_dl(g, cL, sL, dl);
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
-            const C64 Ls = lambda_ls(S, D, r, kin, ev_ls, mkp);
+            const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
             const C64* c = cpl + S->ampcount[r];
 #pragma unroll
             for (int k = 0; k < 2; ++k)
@@ -136,6 +142,10 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
         }
     }
     // A[i][j][m] = e^{i(l1 phiK + l2 phiMu)} dfunhalf(theta_Lb, 2 l0, 2(l1-l2)) dfunone(theta_Jpsi, l2, dmu)
+    const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
+    const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
+    const C64 eK = ld_c(ev, ld, i, E_L_EKR, E_L_EKI);
+    const C64 eM = ld_c(ev, ld, i, E_L_EMR, E_L_EMI);
     const C64 eK3 = eK * eK * eK;
     const C64 ph[6] = {eK * eM, eK, conj(eK), conj(eK) * conj(eM), eK3 * eM, conj(eK3) * conj(eM)};
     const int l2[6] = {1, 0, 0, -1, 1, -1};
@@ -211,16 +221,11 @@ __device__ __forceinline__ ChainAng load_chain(const double* ev, int64_t ld, int
 // Pc angular-times-coupling sums (no lineshape) for resonance r
 // (src/kernel_calEva.cu:610-662; tables of filldlz src/RooDalitzAmplitude.cxx:7189-7245).
 __device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
-                                           const ChainAng& a, const double* cp, const double* sp, double cP,
+                                           const ChainAng& a, double cZ, double sZ, double cP,
                                            double sP, C64 out[8]) {
     const int J2 = S->spin2[r];
     double dz[2][4];
-    switch (J2) {
-    case 1: dz_table<1>(cp, sp, dz); break;
-    case 3: dz_table<3>(cp, sp, dz); break;
-    case 5: dz_table<5>(cp, sp, dz); break;
-    default: dz_table<7>(cp, sp, dz); break;
-    }
+    pc_dz(J2, cZ, sZ, dz);
     C64 V[2][3][2];
     const C64* c = cpl + S->ampcount[r];
     int n = 0;
@@ -241,31 +246,27 @@ __device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, cons
     chain_contract(a, V, out);
 }
 
-// Zcs chain: H += L_X * S_X   (GetAmpZcs, src/kernel_calEva.cu:192-286, with
-// the coupling layout of cuda_configure_ampcount :1066-1087)
+// Zcs chain: H += S_X with S_X = chain_contract(sum_r L_r V_r) (GetAmpZcs,
+// src/kernel_calEva.cu:192-286, coupling layout of cuda_configure_ampcount
+// :1066-1087).  chain_contract is linear in V, so the lineshape-weighted
+// coupling tables of all Zcs resonances are summed first and contracted once;
+// hni += |S_X|^2 (the no-interference sum).
 __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                                          const C64* __restrict__ cpl, const double* __restrict__ ev, int64_t ld,
-                                          int64_t i, double mjk, C64 H[8]) {
-    const ChainAng a = load_chain(ev, ld, i, E_X_CB);
+                                          const C64* __restrict__ cpl, const double* __restrict__ ev,
+                                          const double* __restrict__ gen, int64_t ld, int64_t i, double mjk, C64 H[8],
+                                          double& hni) {
     const double cX = ld_d(ev, ld, i, E_X_CZ), sX = ld_d(ev, ld, i, E_X_SZ);
     const double cP = ld_d(ev, ld, i, E_X_CP), sP = ld_d(ev, ld, i, E_X_SP);
-    double cp[8], sp[8];
-    powers(cX, cp, 7);
-    powers(sX, sp, 7);
     const ChainKin kin = chain_kin(S->P, CH_ZCS, mjk);
     LsEv ev_ls;
     ev_ls.key = -1;
+    C64 V[2][3][2];
     for (int q = 0; q < S->nX; ++q) {
         const int r = S->nR + S->nZ + q;
         const int J2 = S->spin2[r];
         double dx[3][3];
-        switch (J2) {
-        case 0: dx_table<0>(cp, sp, dx); break;
-        case 2: dx_table<2>(cp, sp, dx); break;
-        case 4: dx_table<4>(cp, sp, dx); break;
-        default: dx_table<6>(cp, sp, dx); break;
-        }
-        C64 V[2][3][2];
+        zcs_dx(J2, cX, sX, dx);
+        const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
         const C64* c = cpl + S->ampcount[r];
         int n = 0;
 #pragma unroll
@@ -280,15 +281,18 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
                         const int hz2 = (2 * ip - 1) + (2 * iz - 1);  // 2 helZ = 2 help -+ 1
                         if (hz2 > J2 || hz2 < -J2) continue;
                         const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dx[hz2 / 2 + 1][(1 - ps) + 1];
-                        rmac(V[k][ps][iz], w, c[n]);
+                        rmac(V[k][ps][iz], w, Ls * c[n]);
                         ++n;
                     }
             }
-        C64 Sx[8];
-        chain_contract(a, V, Sx);
-        const C64 Ls = zcs_ls(S, D, r, kin, ev_ls, mjk);
+    }
+    const ChainAng a = load_chain(ev, ld, i, E_X_CB);
+    C64 Sx[8];
+    chain_contract(a, V, Sx);
 #pragma unroll
-        for (int h = 0; h < 8; ++h) cmac(H[h], Ls, Sx[h]);
+    for (int h = 0; h < 8; ++h) {
+        hni += norm2(Sx[h]);
+        H[h] = H[h] + Sx[h];
     }
 }
 

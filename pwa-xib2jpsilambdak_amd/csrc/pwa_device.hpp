@@ -104,8 +104,19 @@ struct StageArgs {
     int shape_blocks, adj_blocks;
     int ncpl, nres, ng;   // gradient layout
     double* gpart_shape;  // shape_blocks x ng (gradient)
-    double* gpart_adj;    // adj_blocks x ng (gradient)
+    double* gpart_adj;    // adj_blocks x ng (gradient, Lambda* / Zcs adjoint)
+    double* gpart_adj_pc; // adj_blocks x ng (gradient, Pc adjoint; NZ > 0)
+    hipEvent_t* tev;      // null, or kStageEvents events recorded around the stage kernels
+    double* gen;          // generic-lineshape buffer (6 ngen planes x ld), null when ngen = 0
+    int ngen;             // Lambda* / Zcs resonances outside the factorised form
 };
+
+// Stage kernels of one segment, in launch order (timing slots).
+enum Stage { ST_LAM = 0, ST_PCANG, ST_SHAPE, ST_ADJ_LAM, ST_ADJ_PC, ST_COUNT };
+constexpr int kStageEvents = ST_COUNT + 1;
+
+hipError_t launch_lam(const StageArgs& a, double* pcs, double* amp, bool store_h0, hipStream_t st);
+hipError_t launch_adj_lam(const StageArgs& a, const double* amp, const double* awp, hipStream_t st);
 
 hipError_t launch_prep(const double* raw, int64_t ld_raw, int64_t n, double* ev, int64_t ld, hipStream_t st);
 hipError_t run_stages(int nz, const StageArgs& a, hipStream_t st);

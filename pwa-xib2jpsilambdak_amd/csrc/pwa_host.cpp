@@ -26,6 +26,8 @@ struct Sample {
     double* ev = nullptr;               // E_NPLANE x ld, device order
     double* fx = nullptr;               // per-event output scratch (device order)
     double* inter = nullptr;            // per-event intermediate planes of the stage pipeline
+    double* gen = nullptr;              // generic-lineshape planes (grown on demand)
+    int64_t gen_planes = 0;
     std::vector<int64_t> perm;          // device position -> caller index
     std::vector<int> seg_scheme;
     std::vector<int64_t> seg_begin, seg_count;
@@ -45,6 +47,7 @@ struct pwa_ctx {
     DevDyn* dD = nullptr;
     C64* dC = nullptr;
     int ncpl = 0;
+    int ngen = 0;  // Lambda* / Zcs lineshapes evaluated by k_lsgen this evaluation
     Sample smp[2];
     Partial* part = nullptr;
     int64_t part_cap = 0;
@@ -62,11 +65,12 @@ struct pwa_ctx {
     float last_ms = 0.f;
     // per-launch timing of the last evaluation
     static constexpr int kMaxLaunch = 32;
-    hipEvent_t le[kMaxLaunch][2] = {};
+    hipEvent_t le[kMaxLaunch][kStageEvents] = {};  // per segment: around each stage kernel
     int nlaunch = 0;
     int32_t l_kind[kMaxLaunch] = {}, l_scheme[kMaxLaunch] = {}, l_grad[kMaxLaunch] = {};
     int64_t l_n[kMaxLaunch] = {};
-    float l_ms[kMaxLaunch] = {};
+    float l_ms[kMaxLaunch] = {};         // whole segment
+    float l_stage[kMaxLaunch][ST_COUNT] = {};  // per stage kernel (enum Stage)
     std::string err;
     std::vector<int> cidx;
     std::vector<double> cmu1, cmu2, cs1, cs2;
@@ -204,9 +208,10 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
 // Per-evaluation parameters: masses/widths, K-matrix blocks (the g_alpha /
 // f_scat packing of ConfigureCouplings and cu_init_data,
 // src/RooDalitzAmplitude.cxx:2831-2916, 3175-3190), pole lists.
-void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
+int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
     const pwa_model_desc& d = c.desc;
     std::memset(&D, 0, sizeof(D));
+    int ngen = 0;
     int nfs[4] = {0, 0, 0, 0};
     for (int r = 0; r < c.nres; ++r) {
         const double* ires = par + param_offset(d, r);
@@ -215,6 +220,7 @@ void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         D.g0[r] = ires[size - 4];
         const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
         D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
+        if (D.ls[r].kind == LS_GENERIC && chain != CH_PC) D.ls[r].slot = ngen++;
         if (d.type[r] != PWA_KMATRIX) continue;
         const int ik = c.hs.ik[r];
         if (D.m0[r] > 0) {
@@ -239,12 +245,22 @@ void build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
             }
         }
     }
+    return ngen;
 }
 
 int upload_params(pwa_ctx* c, const double* params) {
     // the previous evaluation has synchronised the stream, so the pinned
-    // staging buffers are free to overwrite
-    build_dyn(*c, params, *c->hdyn);
+    // staging buffers (and any generic-lineshape buffer) are free to reuse
+    c->ngen = build_dyn(*c, params, *c->hdyn);
+    for (Sample& s : c->smp) {
+        const int64_t need = (int64_t)kGenPlanes * c->ngen;
+        if (!s.present || need <= s.gen_planes) continue;
+        if (s.gen) (void)hipFree(s.gen);
+        s.gen = nullptr;
+        s.gen_planes = 0;
+        HIPCHK(c, hipMalloc(&s.gen, (size_t)need * s.ld * sizeof(double)));
+        s.gen_planes = need;
+    }
     std::vector<cplx> cv(c->ncpl);
     c->cmap.eval(params, cv.data());
     for (int j = 0; j < c->ncpl; ++j) c->hcpl[j] = C64(cv[j].real(), cv[j].imag());
@@ -258,15 +274,20 @@ void collect_times(pwa_ctx* c) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
     for (int q = 0; q < c->nlaunch; ++q) {
+        const int last = c->l_grad[q] ? ST_COUNT : ST_SHAPE + 1;  // stage kernels that ran
         float t = 0.f;
-        c->l_ms[q] = (hipEventElapsedTime(&t, c->le[q][0], c->le[q][1]) == hipSuccess) ? t : -1.f;
+        c->l_ms[q] = (hipEventElapsedTime(&t, c->le[q][0], c->le[q][last]) == hipSuccess) ? t : -1.f;
+        for (int k = 0; k < ST_COUNT; ++k) {
+            t = 0.f;
+            c->l_stage[q][k] = (k < last && hipEventElapsedTime(&t, c->le[q][k], c->le[q][k + 1]) == hipSuccess) ? t : 0.f;
+        }
     }
 }
 
 constexpr int kMaxShapeBlocks = 2048;
 constexpr int kMaxAdjBlocks = 1024;
 
-// Runs the stage pipeline (k_amp, k_shape[, k_adj]) on every segment of both
+// Runs the stage pipeline (k_lam, k_pcang, k_shape[, k_adj_lam, k_adj_pc]) on every segment of both
 // samples, then the reductions; leaves the sums in c->hout (data: s0,c0,s1,c1;
 // MC: s0,c0,s1,c1) and, for the gradient, G_data and G_mc in c->hgout.
 int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool force_unsmeared, bool mc_as_data) {
@@ -274,6 +295,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
     if (rc) return rc;
     const int NG = c->NG;
     struct Seg { int k; size_t q; int sb, ab; };
+    const int n_adj = c->desc.n_pc > 0 ? 2 : 1;  // adjoint kernels per segment
     std::vector<Seg> segs;
     int64_t prow = 0, grow = 0;
     int32_t beg[6] = {0, 0, 0, 0, 0, 0};
@@ -289,7 +311,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
             const int ab = grad ? (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxAdjBlocks) : 0;
             segs.push_back({k, q, sb, ab});
             prow += sb;
-            grow += grad ? sb + ab : 0;
+            grow += grad ? sb + ab * n_adj : 0;
         }
     }
     beg[2] = (int32_t)prow;
@@ -327,6 +349,8 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.D = c->dD;
         a.cpl = c->dC;
         a.inter = s.inter + off;
+        a.gen = s.gen ? s.gen + off : nullptr;
+        a.ngen = c->ngen;
         a.fx = want_fx ? s.fx + off : nullptr;
         a.part = c->part + pr;
         a.shape_blocks = g.sb;
@@ -336,18 +360,18 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.ng = NG;
         a.gpart_shape = grad ? c->gpart + gr * NG : nullptr;
         a.gpart_adj = grad ? c->gpart + (gr + g.sb) * NG : nullptr;
+        a.gpart_adj_pc = grad && n_adj > 1 ? c->gpart + (gr + g.sb + g.ab) * NG : nullptr;
         const int li = c->nlaunch < pwa_ctx::kMaxLaunch ? c->nlaunch++ : -1;
         if (li >= 0) {
-            HIPCHK(c, hipEventRecord(c->le[li][0], c->st));
+            a.tev = c->le[li];
             c->l_kind[li] = g.k;
             c->l_scheme[li] = a.scheme;
             c->l_n[li] = n;
             c->l_grad[li] = grad ? 1 : 0;
         }
         HIPCHK(c, run_stages(c->desc.n_pc, a, c->st));
-        if (li >= 0) HIPCHK(c, hipEventRecord(c->le[li][1], c->st));
         pr += g.sb;
-        gr += grad ? g.sb + g.ab : 0;
+        gr += grad ? g.sb + g.ab * n_adj : 0;
     }
     HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, c->dout, c->st));
@@ -436,7 +460,7 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipEventCreate(&c->e0))) return PWA_E_HIP;
     if (bad(hipEventCreate(&c->e1))) return PWA_E_HIP;
     for (int q = 0; q < pwa_ctx::kMaxLaunch; ++q)
-        for (int e = 0; e < 2; ++e)
+        for (int e = 0; e < kStageEvents; ++e)
             if (bad(hipEventCreate(&c->le[q][e]))) return PWA_E_HIP;
     if (bad(hipMemcpy(c->dS, &c->hs, sizeof(DevStatic), hipMemcpyHostToDevice))) return PWA_E_HIP;
     *out = c;
@@ -451,10 +475,11 @@ int pwa_destroy(pwa_ctx* c) {
         if (c->smp[k].ev) (void)hipFree(c->smp[k].ev);
         if (c->smp[k].fx) (void)hipFree(c->smp[k].fx);
         if (c->smp[k].inter) (void)hipFree(c->smp[k].inter);
+        if (c->smp[k].gen) (void)hipFree(c->smp[k].gen);
     }
-    if (c->dS) hipFree(c->dS);
-    if (c->dD) hipFree(c->dD);
-    if (c->dC) hipFree(c->dC);
+    if (c->dS) (void)hipFree(c->dS);
+    if (c->dD) (void)hipFree(c->dD);
+    if (c->dC) (void)hipFree(c->dC);
     if (c->part) (void)hipFree(c->part);
     if (c->dbeg) (void)hipFree(c->dbeg);
     if (c->dout) (void)hipFree(c->dout);
@@ -467,7 +492,7 @@ int pwa_destroy(pwa_ctx* c) {
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     for (int q = 0; q < pwa_ctx::kMaxLaunch; ++q)
-        for (int e = 0; e < 2; ++e)
+        for (int e = 0; e < kStageEvents; ++e)
             if (c->le[q][e]) (void)hipEventDestroy(c->le[q][e]);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -490,6 +515,7 @@ int pwa_upload_events(pwa_ctx* c, int kind, int64_t n, const double* cols, int64
     if (s.ev) (void)hipFree(s.ev);
     if (s.fx) (void)hipFree(s.fx);
     if (s.inter) (void)hipFree(s.inter);
+    if (s.gen) (void)hipFree(s.gen);
     s = Sample();
     s.n = n;
     s.ld = std::max<int64_t>(((n + 255) / 256) * 256, 256);
@@ -692,7 +718,8 @@ int pwa_last_launches(pwa_ctx* c, int32_t cap, int32_t* n, double* ms, int32_t* 
     if (!c || !n) return PWA_E_ARG;
     *n = c->nlaunch;
     for (int q = 0; q < c->nlaunch && q < cap; ++q) {
-        if (ms) ms[q] = c->l_ms[q];
+        if (ms)
+            for (int k = 0; k < ST_COUNT; ++k) ms[ST_COUNT * q + k] = c->l_stage[q][k];
         if (kind) kind[q] = c->l_kind[q];
         if (scheme) scheme[q] = c->l_scheme[q];
         if (is_grad) is_grad[q] = c->l_grad[q];

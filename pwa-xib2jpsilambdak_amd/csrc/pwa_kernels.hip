@@ -1,8 +1,9 @@
 // HIP kernels (gfx950) for the Lambda_b -> J/psi p K- amplitude likelihood.
 //
 //  k_prep   : raw helicity variables -> compact per-event planes (once per upload)
-//  k_amp, k_shape, k_adj : the per-event stages (pwa_stages.hpp, one
-//             translation unit per Pc count), dispatched by run_stages
+//  k_lam, k_adj_lam (pwa_lambda.hip) and k_pcang, k_shape, k_adj_pc
+//             (pwa_stages.hpp, one translation unit per Pc count): the
+//             per-event stages, dispatched by run_stages
 //  k_reduce, k_grad_reduce : deterministic compensated reductions of the
 //             per-block partials
 //

@@ -1,0 +1,120 @@
+// Lambda* + Zcs stage kernels (independent of the number of Pc-chain
+// resonances, so compiled once):
+//  k_lam     : per event, the coherent Lambda* + Zcs sum H0 (GetAmpLz and
+//              GetAmpZcs, src/kernel_calEva.cu:85-286), |H0|^2 and the
+//              no-interference sum; H0 itself when the Pc stage or the
+//              gradient needs it
+//  k_adj_lam : per event, the adjoint of the Lambda* / Zcs couplings and
+//              lineshape parameters from the seed Y = dF/d(conj A)
+//  k_lsgen   : per event, the Lambda* / Zcs lineshapes whose parameters are
+//              outside the range of the factorised form, with derivatives,
+//              into the generic buffer the two kernels above read
+#include <hip/hip_runtime.h>
+
+#include "pwa_adjoint.hpp"
+
+namespace pwa {
+
+__global__ void __launch_bounds__(kBlock) k_lsgen(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                                 const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                 double* __restrict__ gen) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nr = S->nR + S->nZ + S->nX;
+    for (int r = 0; r < nr; ++r) {
+        const int slot = D->ls[r].slot;
+        if (slot < 0) continue;
+        const double m = r < S->nR ? ld_d(ev, ld, i, E_MKP) : ld_d(ev, ld, i, E_MJPSIK);
+        C64 v, d0, d1;
+        chain_ls_generic(S, D, r, m, v, d0, d1);
+        double* g = gen + (int64_t)(kGenPlanes * slot) * ld + i;
+        g[0] = v.re;
+        g[ld] = v.im;
+        g[2 * ld] = d0.re;
+        g[3 * ld] = d0.im;
+        g[4 * ld] = d1.re;
+        g[5 * ld] = d1.im;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock, 3) k_lam(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                               const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                               const C64* __restrict__ cpl, const double* __restrict__ gen,
+                                               double* __restrict__ pcs, double* __restrict__ amp, int store_h0) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    C64 H0[8];
+    lambda_chain(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MKP), H0);
+    double hni = 0.0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) hni += norm2(H0[h]);
+    if (S->nX > 0) zcs_chain(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MJPSIK), H0, hni);
+    double h02 = 0.0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) h02 += norm2(H0[h]);
+    pcs[i] = h02;
+    pcs[ld + i] = hni;
+    if (store_h0) {
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            amp[(int64_t)(2 * h) * ld + i] = H0[h].re;
+            amp[(int64_t)(2 * h + 1) * ld + i] = H0[h].im;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                                   const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                   const C64* __restrict__ cpl, const double* __restrict__ gen,
+                                                   const double* __restrict__ amp, const double* __restrict__ awp,
+                                                   GradLayout G, int NGp, double* __restrict__ gpart) {
+    extern __shared__ double lds[];
+    Emitter em = Emitter::make(lds, NGp);
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+        const int64_t gi = base + threadIdx.x;
+        const bool valid = gi < n;
+        const int64_t i = valid ? gi : n - 1;
+        auto A = [&](int p) { return amp[(int64_t)p * ld + i]; };
+        const double a2 = valid ? 2.0 * awp[i] : 0.0;
+        {
+            C64 Y[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) Y[h] = C64(A(2 * h), A(2 * h + 1));
+            lambda_adjoint(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MKP), Y, a2, G, em);
+        }
+        if (S->nX > 0) {
+            C64 Y[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) Y[h] = C64(A(2 * h), A(2 * h + 1));
+            zcs_adjoint(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MJPSIK), Y, a2, G, em);
+        }
+    }
+    if (em.cnt > 0) em.flush();
+    __syncthreads();
+    for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+}
+
+hipError_t launch_lam(const StageArgs& a, double* pcs, double* amp, bool store_h0, hipStream_t st) {
+    const unsigned eblocks = (unsigned)((a.n + kBlock - 1) / kBlock);
+    if (a.ngen > 0) {
+        hipLaunchKernelGGL(k_lsgen, dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.gen);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_lam, dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.cpl, a.gen, pcs, amp,
+                       store_h0 ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_lam(const StageArgs& a, const double* amp, const double* awp, hipStream_t st) {
+    const size_t lds = grad_lds_bytes(a.ng);
+    hipError_t e = hipFuncSetAttribute((const void*)k_adj_lam, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    GradLayout G{a.ncpl, a.nres, a.ng};
+    const int NGp = (a.ng + 1) & ~1;
+    hipLaunchKernelGGL(k_adj_lam, dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n, a.S, a.D,
+                       a.cpl, a.gen, amp, awp, G, NGp, a.gpart_adj);
+    return hipGetLastError();
+}
+
+} // namespace pwa

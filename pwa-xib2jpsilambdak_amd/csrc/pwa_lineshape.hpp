@@ -17,7 +17,10 @@
 // is the product of bw_lambda / bw_twobody and vertex_* (pwa_math.hpp) with the
 // m0-dependent factors collected into C and K; the resonances whose branch
 // conditions leave the usual ranges (below-threshold masses, the Sigma pi
-// width, negative masses) keep the direct evaluation (kind LS_GENERIC).
+// width, negative masses) keep the direct evaluation (kind LS_GENERIC): for
+// the Lambda* and Zcs chains in a separate per-event pass (k_lsgen) whose
+// values and derivatives the amplitude kernels read back, for the Pc chain
+// inline.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,10 +30,15 @@
 namespace pwa {
 
 enum LsKind { LS_GENERIC = 0, LS_BW = 1, LS_VERTEX = 2 };
+
+// Planes of the per-event generic-lineshape buffer: for slot k, planes
+// 6k .. 6k+5 hold Re, Im of L and of dL/dm0, dL/dGamma0.
+constexpr int kGenPlanes = 6;
 enum Chain { CH_LAMBDA = 0, CH_PC = 1, CH_ZCS = 2 };
 
 struct LsConst {
-    int32_t kind, LR, LB, pad;
+    int32_t kind, LR, LB;
+    int32_t slot;  // LS_GENERIC Lambda* / Zcs: plane group of the per-event generic buffer
     double m2[3];  // m0^2 and its (m0, Gamma0) derivatives
     double K[3];
     double C[3];
@@ -72,30 +80,33 @@ struct LsEv {
     int key;
     double G, E, GB;
 };
-PWA_HD void ls_event(const Masses& P, const ChainKin& k, int LR, int LB, bool vertex_only, LsEv& e) {
+PWA_HD LsEv ls_event(const Masses& P, const ChainKin& k, int LR, int LB, bool vertex_only) {
     const double rp = P.ffb * k.pB;
     const double bzB = bpoly(rp * rp, LB);
     const double pBL = ipow(k.pB, LB);
+    double G = 0.0, E = 0.0, GB = 0.0;
     if (vertex_only) {
-        e.GB = pBL * sqrt(1.0 / bzB);
-        e.G = 0.0;
-        e.E = 0.0;
+        GB = pBL * sqrt(1.0 / bzB);
     } else {
         const double rr = P.ffr * k.pR;
         const double bz = bpoly(rr * rr, LR);
         const double X = 1.0 / (bz * bzB);
         const double pL = ipow(k.pR, LR);
-        e.G = pL * pBL * sqrt(X);
-        e.E = pL * pL * k.pR * bzB * X * k.imp;
-        e.GB = 0.0;
+        G = pL * pBL * sqrt(X);
+        E = pL * pL * k.pR * bzB * X * k.imp;
     }
+    LsEv e;
     e.key = vertex_only ? (64 + LB) : (LR * 8 + LB);
+    e.G = G;
+    e.E = E;
+    e.GB = GB;
+    return e;
 }
 // Refresh the cached event factors when the orbital momenta change.
 PWA_HD void ls_event_cached(const Masses& P, const ChainKin& k, const LsConst& c, LsEv& e) {
     const bool vo = c.kind == LS_VERTEX;
     const int key = vo ? (64 + c.LB) : (c.LR * 8 + c.LB);
-    if (key != e.key) ls_event(P, k, c.LR, c.LB, vo, e);
+    if (key != e.key) e = ls_event(P, k, c.LR, c.LB, vo);
 }
 
 // Value of a fast lineshape.
@@ -135,6 +146,7 @@ PWA_HD void ls_grad(const LsConst& c, const ChainKin& k, const LsEv& e, C64& v, 
 inline LsConst ls_const(const Masses& P, int chain, int type, int spin2, int LR, double m0v, double g0v) {
     using D = Dual<2>;
     LsConst c{};
+    c.slot = -1;
     c.LR = LR;
     c.LB = vertex_LB(chain, spin2);
     c.kind = LS_GENERIC;

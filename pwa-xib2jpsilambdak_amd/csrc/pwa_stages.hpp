@@ -1,5 +1,7 @@
-// Per-event stage kernels of the likelihood (k_amp, k_shape, k_adj) and the
-// per-segment launcher, templated on the number of Pc-chain resonances NZ.
+// Per-event stage kernels of the likelihood that depend on the number of
+// Pc-chain resonances NZ (k_pcang, k_shape, k_adj_pc) and the per-segment
+// launcher that runs them with the NZ-independent Lambda* / Zcs kernels
+// (pwa_lambda.hip):  k_lam -> k_pcang -> k_shape [-> k_adj_lam -> k_adj_pc].
 // Each NZ is instantiated in its own translation unit (pwa_stages_nz.hip
 // compiled with -DPWA_NZ=k) so the kernels build in parallel.
 #pragma once
@@ -12,80 +14,53 @@ namespace pwa {
 
 // ---------------------------------------------------- intermediate planes
 // pcs (quadratic form):  [H02, HNI, p0 (2 NZ), Md (NZ), Mo (NZ(NZ-1))]  -> 2 + 2 NZ + NZ^2
-// amp (gradient only):   k_amp writes [H0 (16), S_r (16 NZ)]; k_shape overwrites
+// amp:                   k_lam writes H0 (16), k_pcang S_r (16 NZ, gradient); k_shape overwrites
 //                        them in place with the adjoint seeds [Y (16), U_r (16 NZ)]
 // aw  (gradient only):   the event's likelihood weight a (w/f for data, w for MC)
 
-// ------------------------------------------------------------ k_amp
+// ---------------------------------------------------------- k_pcang
+// Pc angular-times-coupling sums S_r (no lineshape) and the quadratic-form
+// coefficients p0_r = sum_h conj(H0_h) S_rh, M_qr = sum_h conj(S_qh) S_rh
+// (src/kernel_calEva.cu:610-662, 786-810); S_r kept for the gradient.
 template <int NZ, bool GRAD>
-__global__ void __launch_bounds__(kBlock) k_amp(const double* __restrict__ ev, int64_t ld, int64_t n,
-                                               const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                                               const C64* __restrict__ cpl, double* __restrict__ pcs,
-                                               double* __restrict__ amp) {
+__global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                                 const DevStatic* __restrict__ S, const C64* __restrict__ cpl,
+                                                 double* __restrict__ pcs, double* __restrict__ amp) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     auto P = [&](int p) -> double& { return pcs[(int64_t)p * ld + i]; };
-    C64 H0[8];
-    lambda_chain(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MKP), H0);
-    double hni = 0.0;
+    const ChainAng a = load_chain(ev, ld, i, E_Z_CB);
+    const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
+    const double cZ = ld_d(ev, ld, i, E_Z_CZ), sZ = ld_d(ev, ld, i, E_Z_SZ);
+    C64 Sr[NZ][8];
 #pragma unroll
-    for (int h = 0; h < 8; ++h) hni += norm2(H0[h]);
-    if (S->nX > 0) {
-        C64 HX[8];
-        zcs_chain(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MJPSIK), HX);
-#pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            hni += norm2(HX[h]);
-            H0[h] = H0[h] + HX[h];
-        }
-    }
-    double h02 = 0.0;
-#pragma unroll
-    for (int h = 0; h < 8; ++h) h02 += norm2(H0[h]);
-    P(0) = h02;
-    P(1) = hni;
-    if constexpr (GRAD) {
+    for (int r = 0; r < NZ; ++r) {
+        pc_angular(S, cpl, S->nR + r, a, cZ, sZ, cP, sP, Sr[r]);
+        C64 p0;
+        double md = 0.0;
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
-            amp[(int64_t)(2 * h) * ld + i] = H0[h].re;
-            amp[(int64_t)(2 * h + 1) * ld + i] = H0[h].im;
+            const C64 h0(amp[(int64_t)(2 * h) * ld + i], amp[(int64_t)(2 * h + 1) * ld + i]);
+            cmac(p0, conj(h0), Sr[r][h]);
+            md += norm2(Sr[r][h]);
         }
-    }
-    if constexpr (NZ > 0) {
-        const ChainAng a = load_chain(ev, ld, i, E_Z_CB);
-        const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
-        double cp[8], sp[8];
-        powers(ld_d(ev, ld, i, E_Z_CZ), cp, 7);
-        powers(ld_d(ev, ld, i, E_Z_SZ), sp, 7);
-        C64 Sr[NZ][8];
+        P(2 + 2 * r) = p0.re;
+        P(3 + 2 * r) = p0.im;
+        P(2 + 2 * NZ + r) = md;
 #pragma unroll
-        for (int r = 0; r < NZ; ++r) {
-            pc_angular(S, cpl, S->nR + r, a, cp, sp, cP, sP, Sr[r]);
-            C64 p0;
-            double md = 0.0;
+        for (int q = 0; q < r; ++q) {
+            C64 m;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) cmac(m, conj(Sr[q][h]), Sr[r][h]);
+            const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
+            P(o) = m.re;
+            P(o + 1) = m.im;
+        }
+        if constexpr (GRAD) {
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
-                cmac(p0, conj(H0[h]), Sr[r][h]);
-                md += norm2(Sr[r][h]);
-            }
-            P(2 + 2 * r) = p0.re;
-            P(3 + 2 * r) = p0.im;
-            P(2 + 2 * NZ + r) = md;
-#pragma unroll
-            for (int q = 0; q < r; ++q) {
-                C64 m;
-#pragma unroll
-                for (int h = 0; h < 8; ++h) cmac(m, conj(Sr[q][h]), Sr[r][h]);
-                const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
-                P(o) = m.re;
-                P(o + 1) = m.im;
-            }
-            if constexpr (GRAD) {
-#pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    amp[(int64_t)(16 + 16 * r + 2 * h) * ld + i] = Sr[r][h].re;
-                    amp[(int64_t)(16 + 16 * r + 2 * h + 1) * ld + i] = Sr[r][h].im;
-                }
+                amp[(int64_t)(16 + 16 * r + 2 * h) * ld + i] = Sr[r][h].re;
+                amp[(int64_t)(16 + 16 * r + 2 * h + 1) * ld + i] = Sr[r][h].im;
             }
         }
     }
@@ -311,16 +286,14 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
     }
 }
 
-// ------------------------------------------------------------- k_adj
-// Coupling adjoint from the seeds k_shape leaves in the amp planes: Y for the
-// Lambda* / Zcs couplings and lineshapes, U_r for the Pc couplings; every
-// contribution weighted by 2 a.
+// ----------------------------------------------------------- k_adj_pc
+// Pc coupling adjoint from the seeds U_r that k_shape leaves in the amp
+// planes; every contribution weighted by 2 a.
 template <int NZ>
-__global__ void __launch_bounds__(kBlock) k_adj(const double* __restrict__ ev, int64_t ld, int64_t n,
-                                               const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                                               const C64* __restrict__ cpl, const double* __restrict__ amp,
-                                               const double* __restrict__ awp, GradLayout G, int NGp,
-                                               double* __restrict__ gpart) {
+__global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__ ev, int64_t ld, int64_t n,
+                                                  const DevStatic* __restrict__ S, const double* __restrict__ amp,
+                                                  const double* __restrict__ awp, GradLayout G, int NGp,
+                                                  double* __restrict__ gpart) {
     extern __shared__ double lds[];
     Emitter em = Emitter::make(lds, NGp);
     for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
@@ -329,33 +302,21 @@ __global__ void __launch_bounds__(kBlock) k_adj(const double* __restrict__ ev, i
         const int64_t i = valid ? gi : n - 1;
         auto A = [&](int p) { return amp[(int64_t)p * ld + i]; };
         const double a2 = valid ? 2.0 * awp[i] : 0.0;
-        {
-            C64 Y[8];
-#pragma unroll
-            for (int h = 0; h < 8; ++h) Y[h] = C64(A(2 * h), A(2 * h + 1));
-            lambda_adjoint(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MKP), Y, a2, G, em);
-            if (S->nX > 0) zcs_adjoint(S, D, cpl, ev, ld, i, ld_d(ev, ld, i, E_MJPSIK), Y, a2, G, em);
-        }
-        if constexpr (NZ > 0) {
-            const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
-            const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
-            double cp[8], sp[8];
-            powers(ld_d(ev, ld, i, E_Z_CZ), cp, 7);
-            powers(ld_d(ev, ld, i, E_Z_SZ), sp, 7);
+        const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
+        const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
+        const double cZ = ld_d(ev, ld, i, E_Z_CZ), sZ = ld_d(ev, ld, i, E_Z_SZ);
 #pragma unroll 1
-            for (int r = 0; r < NZ; ++r) {
-                C64 U[8];
+        for (int r = 0; r < NZ; ++r) {
+            C64 U[8];
 #pragma unroll
-                for (int h = 0; h < 8; ++h) U[h] = C64(A(16 + 16 * r + 2 * h), A(17 + 16 * r + 2 * h));
-                pc_coupling_adjoint(S, S->nR + r, za, cp, sp, cP, sP, U, a2, G, em);
-            }
+            for (int h = 0; h < 8; ++h) U[h] = C64(A(16 + 16 * r + 2 * h), A(17 + 16 * r + 2 * h));
+            pc_coupling_adjoint(S, S->nR + r, za, cZ, sZ, cP, sP, U, a2, G, em);
         }
     }
     if (em.cnt > 0) em.flush();
     __syncthreads();
     for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
 }
-
 
 template <int NZ>
 hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
@@ -367,9 +328,16 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
     const int NGp = (a.ng + 1) & ~1;
     const size_t lds = a.grad ? grad_lds_bytes(a.ng) : 0;
     hipError_t e;
-    if (a.grad) hipLaunchKernelGGL((k_amp<NZ, true>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.cpl, pcs, amp);
-    else hipLaunchKernelGGL((k_amp<NZ, false>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.D, a.cpl, pcs, amp);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    auto mark = [&](int k) { return a.tev ? hipEventRecord(a.tev[k], st) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
+    if ((e = launch_lam(a, pcs, amp, a.grad || NZ > 0, st)) != hipSuccess) return e;
+    if ((e = mark(1)) != hipSuccess) return e;
+    if constexpr (NZ > 0) {
+        if (a.grad) hipLaunchKernelGGL((k_pcang<NZ, true>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.cpl, pcs, amp);
+        else hipLaunchKernelGGL((k_pcang<NZ, false>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.cpl, pcs, amp);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if ((e = mark(2)) != hipSuccess) return e;
     const unsigned sblocks = (unsigned)a.shape_blocks;
 #define PWA_SHAPE(MC, GR, KM)                                                                                      \
     do {                                                                                                           \
@@ -394,12 +362,18 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
     }
 #undef PWA_SHAPE
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess) return e;
     if (a.grad) {
-        e = hipFuncSetAttribute((const void*)k_adj<NZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_adj<NZ>), dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n, a.S,
-                           a.D, a.cpl, amp, awp, G, NGp, a.gpart_adj);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_adj_lam(a, amp, awp, st)) != hipSuccess) return e;
+        if ((e = mark(4)) != hipSuccess) return e;
+        if constexpr (NZ > 0) {
+            e = hipFuncSetAttribute((const void*)k_adj_pc<NZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_adj_pc<NZ>), dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,
+                               a.S, amp, awp, G, NGp, a.gpart_adj_pc);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        if ((e = mark(5)) != hipSuccess) return e;
     }
     return hipSuccess;
 }

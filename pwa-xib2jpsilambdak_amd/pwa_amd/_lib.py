@@ -196,18 +196,27 @@ class Context:
         check(lib().pwa_amp2(self._h, kind, dptr(p), dptr(out)), self._h, "pwa_amp2")
         return out
 
+    STAGES = ("lam", "pcang", "shape", "adj_lam", "adj_pc")
+
     def last_launches(self):
-        """Per-launch (ms, kind, scheme, is_grad, n_events) of the last evaluation."""
+        """Per-segment kernel times of the last evaluation: dicts with ms_<stage>
+        for each of STAGES (device ms, 0 when the kernel did not run), ms (their
+        sum), kind, scheme, grad, n (events)."""
         cap = 32
+        ns = len(self.STAGES)
         n = C.c_int32()
-        ms = np.zeros(cap)
+        ms = np.zeros(ns * cap)
         kind, sch, gr = (np.zeros(cap, np.int32) for _ in range(3))
         ne = np.zeros(cap, np.int64)
         check(lib().pwa_last_launches(self._h, cap, C.byref(n), dptr(ms), iptr(kind), iptr(sch), iptr(gr),
                                       ne.ctypes.data_as(C.POINTER(C.c_int64))), self._h, "pwa_last_launches")
-        k = min(n.value, cap)
-        return [dict(ms=float(ms[q]), kind=int(kind[q]), scheme=int(sch[q]), grad=bool(gr[q]), n=int(ne[q]))
-                for q in range(k)]
+        out = []
+        for q in range(min(n.value, cap)):
+            d = {"ms_" + st: float(ms[ns * q + k]) for k, st in enumerate(self.STAGES)}
+            d.update(ms=float(ms[ns * q:ns * q + ns].sum()), kind=int(kind[q]), scheme=int(sch[q]), grad=bool(gr[q]),
+                     n=int(ne[q]))
+            out.append(d)
+        return out
 
     def last_timing(self):
         t = C.c_double()
