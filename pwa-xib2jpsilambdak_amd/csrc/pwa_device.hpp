@@ -123,7 +123,9 @@ hipError_t run_stages(int nz, const StageArgs& a, hipStream_t st);
 template <int NZ>
 hipError_t run_stages_nz(const StageArgs& a, hipStream_t st);  // one instantiation per NZ = 0..8
 hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, double* out, hipStream_t st);
-hipError_t launch_grad_reduce(const double* gpart, const int32_t* beg, int NG, double* out, hipStream_t st);
+hipError_t launch_grad_reduce(const double* gpart, int b0, int b1, int b2, int NG, double* scratch, double* out,
+                              hipStream_t st);
+size_t grad_reduce_scratch(int rows_data, int rows_mc, int NG);
 size_t grad_lds_bytes(int NG);
 int inter_planes(int nz);
 constexpr int kBlock = 256;

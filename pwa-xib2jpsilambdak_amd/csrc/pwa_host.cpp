@@ -323,11 +323,13 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         HIPCHK(c, hipMalloc(&c->part, prow * sizeof(Partial)));
         c->part_cap = prow;
     }
-    if (grad && grow * NG > c->gpart_cap) {
+    // gradient partial rows, then the chunk sums of the two-pass reduction
+    const int64_t gneed = grow * NG + (int64_t)(grad_reduce_scratch(beg[4] - beg[3], beg[5] - beg[4], NG) / sizeof(double));
+    if (grad && gneed > c->gpart_cap) {
         if (c->gpart) (void)hipFree(c->gpart);
         c->gpart = nullptr;
-        HIPCHK(c, hipMalloc(&c->gpart, std::max<int64_t>(grow, 1) * NG * sizeof(double)));
-        c->gpart_cap = grow * NG;
+        HIPCHK(c, hipMalloc(&c->gpart, std::max<int64_t>(gneed, 1) * sizeof(double)));
+        c->gpart_cap = gneed;
     }
     const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
     c->nlaunch = 0;
@@ -375,7 +377,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
     }
     HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, c->dout, c->st));
-    if (grad) HIPCHK(c, launch_grad_reduce(c->gpart, c->dbeg + 3, NG, c->dgout, c->st));
+    if (grad) HIPCHK(c, launch_grad_reduce(c->gpart, beg[3], beg[4], beg[5], NG, c->gpart + grow * NG, c->dgout, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
     HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     if (grad) HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, 2 * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));

@@ -129,29 +129,65 @@ hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, d
     return hipGetLastError();
 }
 
-// out[k * NG + q] = sum over the partial rows [beg[k], beg[k+1]) of gpart[row * NG + q];
-// 8 row-slices per component reduced in parallel, then combined in order.
-__global__ void __launch_bounds__(kBlock) k_grad_reduce(const double* __restrict__ gpart, const int32_t* __restrict__ beg,
-                                                         int NG, double* __restrict__ out) {
-    __shared__ double red[8][32];
-    const int q = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int slice = threadIdx.x >> 5;
-    const int k = blockIdx.y;
-    const int lo = beg[k], hi = beg[k + 1];
+// Gradient partials: rows [b0, b1) belong to the data sample, [b1, b2) to
+// MC; out[k * NG + q] = sum of column q over the rows of sample k.  Two
+// deterministic passes: pass 1 sums chunks of kGradChunk rows (64 columns x 4
+// row-slices per block, slices combined in order), pass 2 sums the chunk
+// results of each sample in order.
+constexpr int kGradChunk = 128;
+
+__host__ __device__ inline int grad_chunks(int rows) { return (rows + kGradChunk - 1) / kGradChunk; }
+
+__global__ void __launch_bounds__(kBlock) k_grad_chunks(const double* __restrict__ gpart, int b0, int b1, int b2,
+                                                         int NG, double* __restrict__ chunk_out) {
+    __shared__ double red[4][64][2];
+    const int col = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int slice = threadIdx.x >> 6;
+    const int y = blockIdx.y;
+    const int n0 = grad_chunks(b1 - b0);
+    const int lo = (y < n0) ? b0 + y * kGradChunk : b1 + (y - n0) * kGradChunk;
+    const int hi = min(lo + kGradChunk, (y < n0) ? b1 : b2);
     double s = 0.0, c = 0.0;
-    if (q < NG)
-        for (int row = lo + slice; row < hi; row += 8) kadd(s, c, gpart[(int64_t)row * NG + q]);
-    red[slice][threadIdx.x & 31] = s + c;
+    if (col < NG)
+        for (int row = lo + slice; row < hi; row += 4) kadd(s, c, gpart[(int64_t)row * NG + col]);
+    red[slice][threadIdx.x & 63][0] = s;
+    red[slice][threadIdx.x & 63][1] = c;
     __syncthreads();
-    if (slice == 0 && q < NG) {
+    if (slice == 0 && col < NG) {
         double t = 0.0, tc = 0.0;
-        for (int z = 0; z < 8; ++z) kadd(t, tc, red[z][threadIdx.x]);
-        out[k * NG + q] = t + tc;
+        for (int z = 0; z < 4; ++z) {
+            kadd(t, tc, red[z][threadIdx.x][0]);
+            tc += red[z][threadIdx.x][1];
+        }
+        chunk_out[(int64_t)y * NG + col] = t + tc;
     }
 }
 
-hipError_t launch_grad_reduce(const double* gpart, const int32_t* beg, int NG, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_grad_reduce, dim3((NG + 31) / 32, 2), dim3(kBlock), 0, st, gpart, beg, NG, out);
+__global__ void __launch_bounds__(kBlock) k_grad_final(const double* __restrict__ chunk_out, int n0, int n1, int NG,
+                                                        double* __restrict__ out) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y;
+    if (q >= NG) return;
+    const int lo = k == 0 ? 0 : n0, hi = k == 0 ? n0 : n0 + n1;
+    double t = 0.0, tc = 0.0;
+    for (int y = lo; y < hi; ++y) kadd(t, tc, chunk_out[(int64_t)y * NG + q]);
+    out[k * NG + q] = t + tc;
+}
+
+size_t grad_reduce_scratch(int rows_data, int rows_mc, int NG) {
+    return (size_t)(grad_chunks(rows_data) + grad_chunks(rows_mc)) * NG * sizeof(double);
+}
+
+hipError_t launch_grad_reduce(const double* gpart, int b0, int b1, int b2, int NG, double* scratch, double* out,
+                              hipStream_t st) {
+    const int n0 = grad_chunks(b1 - b0), n1 = grad_chunks(b2 - b1);
+    if (n0 + n1 > 0) {
+        hipLaunchKernelGGL(k_grad_chunks, dim3((NG + 63) / 64, n0 + n1), dim3(kBlock), 0, st, gpart, b0, b1, b2, NG,
+                           scratch);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_grad_final, dim3((NG + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, st, scratch, n0, n1, NG, out);
     return hipGetLastError();
 }
 
