@@ -305,13 +305,16 @@ __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
 }
 
 // Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.
-template <int NZ, bool KM = true>
+// FULL: the model has K-matrix members or Pc lineshapes outside the
+// factorised range this evaluation (else only factorised Breit-Wigner /
+// vertex terms occur and the other code is left out).
+template <int NZ, bool FULL = true>
 __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                               double m, C64 L[NZ]) {
     const int nR = S->nR;
 #pragma unroll
     for (int r = 0; r < NZ; ++r) L[r] = C64(1.0, 0.0);
-    if constexpr (KM)
+    if constexpr (FULL)
     for (int c = 0; c < S->km_nclass; ++c) {
         const int ik = S->km_class[c];
         C64 Mtr[8];
@@ -327,16 +330,18 @@ __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, c
     for (int r = 0; r < NZ; ++r) {
         const int rr = nR + r;
         const LsConst& lc = D->ls[rr];
-        if (lc.kind != LS_GENERIC) {
+        if (!FULL || lc.kind != LS_GENERIC) {
             ls_event_cached(S->P, kin, lc, e);
             const C64 v = ls_value(lc, kin, e);
             L[r] = (lc.kind == LS_BW) ? v : L[r] * v.re;
             continue;
         }
-        const double m0 = D->m0[rr];
-        if (S->type[rr] == PWA_BREITWIGNER)
-            L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);
-        L[r] = L[r] * vertex_pc<double>(S->P, S->spin2[rr], m, m0);
+        if constexpr (FULL) {
+            const double m0 = D->m0[rr];
+            if (S->type[rr] == PWA_BREITWIGNER)
+                L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);
+            L[r] = L[r] * vertex_pc<double>(S->P, S->spin2[rr], m, m0);
+        }
     }
 }
 

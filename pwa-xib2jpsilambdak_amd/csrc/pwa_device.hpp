@@ -93,7 +93,7 @@ struct StageArgs {
     int64_t ld, n;
     int scheme;
     bool mc, grad;
-    bool km;              // the model has K-matrix members
+    bool full;            // K-matrix members or generic Pc lineshapes present (full k_shape variant)
     int noint;
     const DevStatic* S;
     const DevDyn* D;

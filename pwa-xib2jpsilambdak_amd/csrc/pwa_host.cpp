@@ -47,7 +47,8 @@ struct pwa_ctx {
     DevDyn* dD = nullptr;
     C64* dC = nullptr;
     int ncpl = 0;
-    int ngen = 0;  // Lambda* / Zcs lineshapes evaluated by k_lsgen this evaluation
+    int ngen = 0;             // Lambda* / Zcs lineshapes evaluated by k_lsgen this evaluation
+    bool pc_generic = false;  // a Pc lineshape is outside the factorised range this evaluation
     Sample smp[2];
     Partial* part = nullptr;
     int64_t part_cap = 0;
@@ -152,7 +153,7 @@ int validate(const pwa_model_desc& d, std::string& err) {
 
 void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) {
     std::memset(&S, 0, sizeof(S));
-    S.P = Masses{d.m_lb, d.m_p, d.m_k, d.m_jpsi, d.ffr, d.ffb};
+    S.P = Masses{d.m_lb, d.m_p, d.m_k, d.m_jpsi, d.ffr, d.ffb, 0.5 / d.m_lb};
     S.nR = d.n_lambda;
     S.nZ = d.n_pc;
     S.nX = d.n_zcs;
@@ -252,6 +253,9 @@ int upload_params(pwa_ctx* c, const double* params) {
     // the previous evaluation has synchronised the stream, so the pinned
     // staging buffers (and any generic-lineshape buffer) are free to reuse
     c->ngen = build_dyn(*c, params, *c->hdyn);
+    c->pc_generic = false;
+    for (int r = c->desc.n_lambda; r < c->desc.n_lambda + c->desc.n_pc; ++r)
+        if (c->hdyn->ls[r].kind == LS_GENERIC) c->pc_generic = true;
     for (Sample& s : c->smp) {
         const int64_t need = (int64_t)kGenPlanes * c->ngen;
         if (!s.present || need <= s.gen_planes) continue;
@@ -345,7 +349,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.scheme = force_unsmeared ? 0 : s.seg_scheme[g.q];
         a.mc = (g.k == PWA_MC) && !mc_as_data;
         a.grad = grad;
-        a.km = c->hs.km_nclass > 0;
+        a.full = c->hs.km_nclass > 0 || c->pc_generic;
         a.noint = noint;
         a.S = c->dS;
         a.D = c->dD;

@@ -51,28 +51,31 @@ PWA_HD int vertex_LB(int chain, int spin2) {
     return spin2 >= 1 ? (spin2 - 1) / 2 : 0;
 }
 
-// Per-event kinematics of a chain at resonance mass m.
+// Per-event kinematics of a chain at resonance mass m: s = m^2, 1/m, the
+// breakup momenta pR (resonance -> daughters) and pB (Lambda_b -> resonance
+// + bachelor), as breakup_d with the divisions by 2m and 2 m_lb folded into
+// one reciprocal and a constant.
 struct ChainKin {
     double s, imp, pR, pB;
 };
 PWA_HD ChainKin chain_kin(const Masses& P, int chain, double m) {
+    double a1, a2, b;  // resonance daughters, Lambda_b bachelor
+    if (chain == CH_LAMBDA) { a1 = P.m_p; a2 = P.m_k; b = P.m_jpsi; }
+    else if (chain == CH_PC) { a1 = P.m_p; a2 = P.m_jpsi; b = P.m_k; }
+    else { a1 = P.m_k; a2 = P.m_jpsi; b = P.m_p; }
     ChainKin k;
     k.s = m * m;
-    k.imp = 1.0 / m;
-    if (chain == CH_LAMBDA) {
-        k.pR = breakup_d(m, P.m_p, P.m_k);
-        k.pB = breakup_d(P.m_lb, P.m_jpsi, m);
-    } else if (chain == CH_PC) {
-        k.pR = breakup_d(m, P.m_p, P.m_jpsi);
-        k.pB = breakup_d(P.m_lb, P.m_k, m);
-    } else {
-        k.pR = breakup_d(m, P.m_k, P.m_jpsi);
-        k.pB = breakup_d(P.m_lb, P.m_p, m);
-    }
+    k.imp = rcp_nr(m);
+    const double u = (k.s - (a1 + a2) * (a1 + a2)) * (k.s - (a1 - a2) * (a1 - a2));
+    k.pR = sqrt_nr(fabs(u)) * (0.5 * k.imp);
+    const double mlb2 = P.m_lb * P.m_lb;
+    const double v = (mlb2 - (b + m) * (b + m)) * (mlb2 - (b - m) * (b - m));
+    k.pB = sqrt_nr(fabs(v)) * P.half_inv_mlb;
     return k;
 }
 
-PWA_HD double bpoly(double z, int L) { return (L >= 1 && L <= 5) ? bw_poly<double>(z, L) : 1.0; }
+PWA_HD bool has_barrier(int L) { return L >= 1 && L <= 5; }
+PWA_HD double bpoly(double z, int L) { return has_barrier(L) ? bw_poly<double>(z, L) : 1.0; }
 
 // Event factors for orbital momenta (LR, LB): G and E as above, and the
 // vertex-only factor GB = pB^LB / sqrt(B_LB(zB)).
@@ -81,19 +84,24 @@ struct LsEv {
     double G, E, GB;
 };
 PWA_HD LsEv ls_event(const Masses& P, const ChainKin& k, int LR, int LB, bool vertex_only) {
-    const double rp = P.ffb * k.pB;
-    const double bzB = bpoly(rp * rp, LB);
     const double pBL = ipow(k.pB, LB);
+    double ibzB = 1.0;
+    if (has_barrier(LB)) {
+        const double rp = P.ffb * k.pB;
+        ibzB = rcp_nr(bw_poly<double>(rp * rp, LB));
+    }
     double G = 0.0, E = 0.0, GB = 0.0;
     if (vertex_only) {
-        GB = pBL * sqrt(1.0 / bzB);
+        GB = has_barrier(LB) ? pBL * sqrt_nr(ibzB) : pBL;
     } else {
-        const double rr = P.ffr * k.pR;
-        const double bz = bpoly(rr * rr, LR);
-        const double X = 1.0 / (bz * bzB);
+        double ibz = 1.0;
+        if (has_barrier(LR)) {
+            const double rr = P.ffr * k.pR;
+            ibz = rcp_nr(bw_poly<double>(rr * rr, LR));
+        }
         const double pL = ipow(k.pR, LR);
-        G = pL * pBL * sqrt(X);
-        E = pL * pL * k.pR * bzB * X * k.imp;
+        G = (has_barrier(LR) || has_barrier(LB)) ? pL * pBL * sqrt_nr(ibz * ibzB) : pL * pBL;
+        E = pL * pL * k.pR * ibz * k.imp;
     }
     LsEv e;
     e.key = vertex_only ? (64 + LB) : (LR * 8 + LB);
@@ -114,7 +122,7 @@ PWA_HD C64 ls_value(const LsConst& c, const ChainKin& k, const LsEv& e) {
     if (c.kind == LS_VERTEX) return C64(c.C[0] * e.GB, 0.0);
     const double a = c.m2[0] - k.s;
     const double b = c.K[0] * e.E;
-    const double t = c.C[0] * e.G / (a * a + b * b);
+    const double t = c.C[0] * e.G * rcp_nr(a * a + b * b);
     return C64(a * t, b * t);
 }
 
@@ -128,7 +136,7 @@ PWA_HD void ls_grad(const LsConst& c, const ChainKin& k, const LsEv& e, C64& v, 
     }
     const double a = c.m2[0] - k.s;
     const double b = c.K[0] * e.E;
-    const double inv = 1.0 / (a * a + b * b);
+    const double inv = rcp_nr(a * a + b * b);
     const C64 w(a * inv, b * inv);  // 1 / (a - i b)
     const C64 w2 = w * w;
     const double cg = c.C[0] * e.G;

@@ -104,6 +104,38 @@ PWA_HD double dsqrt(double a) { return sqrt(a); }
 PWA_HD double dtanh(double a) { return tanh(a); }
 PWA_HD double dfabs(double a) { return fabs(a); }
 
+// Reciprocal and square root for the per-event hot paths: the hardware
+// approximations v_rcp_f64 / v_rsq_f64 refined by two Newton steps
+// (reciprocal) or one Goldschmidt step plus a residual correction (square
+// root).  For positive normal arguments they agree with IEEE 1/x and sqrt(x)
+// to the last bit on the sample of tests/cpp/fastmath_check.hip (2^20
+// arguments over 1e-13 .. 1e13); sqrt_nr(0) = 0.  Host code uses the IEEE
+// operations.
+PWA_HD double rcp_nr(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+PWA_HD double sqrt_nr(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(h, fma(-g, g, x), g);
+    return x > 0.0 ? g : 0.0;
+#else
+    return sqrt(x);
+#endif
+}
+
 // x^n for a small non-negative integer n (exact multiplication chain).
 template <class T> PWA_HD T ipow(T x, int n) {
     T r(1.0);
@@ -259,6 +291,7 @@ PWA_HD C64 csqrt_branch(C64 x) {
 // ------------------------------------------------------------- lineshapes
 struct Masses {
     double m_lb, m_p, m_k, m_jpsi, ffr, ffb;
+    double half_inv_mlb;  // 1 / (2 m_lb)
 };
 
 // 1/(m^2 - s - i m Gamma)

@@ -68,7 +68,9 @@ __global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev,
 
 // ----------------------------------------------------------- k_shape
 // Pc lineshapes over the event's sub-bins and the per-event likelihood term.
-template <int NZ, bool MC, bool GRAD, bool KM>
+// FULL: K-matrix members or Pc lineshapes outside the factorised range are
+// present this evaluation (pc_lineshapes).
+template <int NZ, bool MC, bool GRAD, bool FULL>
 __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev, int64_t ld, int64_t n, int scheme,
                                                  const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                  const double* __restrict__ pcs, double* __restrict__ amp,
@@ -130,7 +132,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             for (int b = 0; b < nb; ++b) {
                 const double mb = mjp + S->scheme_z[scheme][b] * sig;
                 if constexpr (!GRAD) {
-                    pc_lineshapes<NZ, KM>(S, D, mb, L);
+                    pc_lineshapes<NZ, FULL>(S, D, mb, L);
                     acc += pc_form<NZ>(L, p0, Md, Mo);
                 } else {
                     C64 Mt[NZ], dL0[NZ], dL1[NZ];
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                     KMState st;
 #pragma unroll
                     for (int r = 0; r < NZ; ++r) Mt[r] = C64(1.0, 0.0);
-                    if constexpr (KM)
+                    if constexpr (FULL)
                     for (int c = 0; c < S->km_nclass; ++c) {
                         const int ik = S->km_class[c];
                         C64 Mtr[8];
@@ -152,7 +154,8 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                     LsEv e;
                     e.key = -1;
 #pragma unroll
-                    for (int r = 0; r < NZ; ++r) pc_ls_grad(S, D, S->nR + r, kin, e, mb, Mt[r], L[r], dL0[r], dL1[r], Vx[r]);
+                    for (int r = 0; r < NZ; ++r)
+                        pc_ls_grad<FULL>(S, D, S->nR + r, kin, e, mb, Mt[r], L[r], dL0[r], dL1[r], Vx[r]);
                     acc += pc_form<NZ>(L, p0, Md, Mo);
                     C64 Ab[8];
 #pragma unroll
@@ -178,13 +181,13 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                                 if (q == S->ikm[S->nR + r]) Ab[q] = Ab[q] + add;
                         }
                     }
-                    if constexpr (KM)
+                    if constexpr (FULL)
                         if (ikc >= 0) kmatrix_adjoint(D, ikc, st, Ab, kgb, kfb, kmb);
                 }
             }
             f += acc * inb;
             if (MC && noint) {
-                if (scheme != 0) pc_lineshapes<NZ, KM>(S, D, mjp, L);
+                if (scheme != 0) pc_lineshapes<NZ, FULL>(S, D, mjp, L);
                 C64 zero[NZ];
                 fni += pc_form<NZ>(L, zero, Md, Mo);
             }
@@ -256,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                     em.put(2 * r + 1, sc * gw[r]);
                 }
                 em.commit(2 * NZ, G.res(S->nR));
-                if (KM && ikc >= 0) {
+                if (FULL && ikc >= 0) {
                     const int o = G.km(ikc);
                     em.begin(16);
 #pragma unroll
@@ -339,17 +342,17 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
     }
     if ((e = mark(2)) != hipSuccess) return e;
     const unsigned sblocks = (unsigned)a.shape_blocks;
-#define PWA_SHAPE(MC, GR, KM)                                                                                      \
+#define PWA_SHAPE(MC, GR, FL)                                                                                      \
     do {                                                                                                           \
         if (GR) {                                                                                                  \
-            e = hipFuncSetAttribute((const void*)k_shape<NZ, MC, GR, KM>,                                          \
+            e = hipFuncSetAttribute((const void*)k_shape<NZ, MC, GR, FL>,                                          \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                         \
             if (e != hipSuccess) return e;                                                                         \
         }                                                                                                          \
-        hipLaunchKernelGGL((k_shape<NZ, MC, GR, KM>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,        \
+        hipLaunchKernelGGL((k_shape<NZ, MC, GR, FL>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,        \
                            a.scheme, a.S, a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart_shape);            \
     } while (0)
-    if (a.km) {
+    if (a.full) {
         if (a.mc && a.grad) PWA_SHAPE(true, true, true);
         else if (a.mc) PWA_SHAPE(true, false, true);
         else if (a.grad) PWA_SHAPE(false, true, true);

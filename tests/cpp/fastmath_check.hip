@@ -1,0 +1,64 @@
+// GPU check of rcp_nr / sqrt_nr (pwa_math.hpp) against IEEE 1/x and sqrt(x):
+// 2^20 log-uniform arguments over 1e-13 .. 1e13 plus a few exact squares and
+// powers of two; prints the largest difference in ulp and exits non-zero if
+// either exceeds 1 ulp.  Run by tests/test_gpu_fastmath.py.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pwa_math.hpp"
+
+__global__ void k_check(const double* __restrict__ x, double* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    out[i] = pwa::rcp_nr(v);
+    out[n + i] = pwa::sqrt_nr(v);
+    out[2 * n + i] = 1.0 / v;
+    out[3 * n + i] = sqrt(v);
+}
+
+static int64_t ulps(double a, double b) {
+    int64_t ia, ib;
+    std::memcpy(&ia, &a, 8);
+    std::memcpy(&ib, &b, 8);
+    return ia > ib ? ia - ib : ib - ia;
+}
+
+int main() {
+    const int n = 1 << 20;
+    std::vector<double> x(n), o(4 * (size_t)n);
+    uint64_t s = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < n; ++i) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        const double u = (double)(s >> 11) * (1.0 / 9007199254740992.0);
+        x[i] = std::exp((u - 0.5) * 60.0);
+    }
+    for (int i = 0; i < 64; ++i) x[i] = (double)((i + 1) * (i + 1));
+    for (int i = 64; i < 128; ++i) x[i] = std::ldexp(1.0, i - 96);
+    double *dx = nullptr, *dout = nullptr;
+    if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess) return 3;
+    if (hipMalloc(&dout, 4 * (size_t)n * sizeof(double)) != hipSuccess) return 3;
+    if (hipMemcpy(dx, x.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 3;
+    hipLaunchKernelGGL(k_check, dim3(n / 256), dim3(256), 0, 0, dx, dout, n);
+    if (hipDeviceSynchronize() != hipSuccess) return 3;
+    if (hipMemcpy(o.data(), dout, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+    int64_t mr = 0, ms = 0, dr = 0, ds = 0;
+    for (int i = 0; i < n; ++i) {
+        mr = std::max(mr, ulps(o[i], 1.0 / x[i]));
+        ms = std::max(ms, ulps(o[n + i], std::sqrt(x[i])));
+        dr = std::max(dr, ulps(o[2 * (size_t)n + i], 1.0 / x[i]));
+        ds = std::max(ds, ulps(o[3 * (size_t)n + i], std::sqrt(x[i])));
+    }
+    std::printf("rcp_nr max ulp %lld, sqrt_nr max ulp %lld (device IEEE: %lld, %lld)\n", (long long)mr, (long long)ms,
+                (long long)dr, (long long)ds);
+    (void)hipFree(dx);
+    (void)hipFree(dout);
+    return (mr <= 1 && ms <= 1) ? 0 : 1;
+}

@@ -4,7 +4,7 @@
 using namespace pwa;
 int main() {
     DevStatic S{}; DevDyn D{};
-    S.P = Masses{5.6196, 0.938272, 0.493677, 3.0969, 3.0, 3.0};
+    S.P = Masses{5.6196, 0.938272, 0.493677, 3.0969, 3.0, 3.0, 0.5 / 5.6196};
     S.flags = 0;
     double ma[4] = {0.938272, 1.86484, 2.00685, 1.86966}, mb[4] = {3.0969, 2.45265, 2.45265, 2.45397}, hw[4] = {0, 0.00115, 0.00115, 0.000945};
     for (int c = 0; c < 4; ++c) { S.km_ma[0][c] = ma[c]; S.km_mb[0][c] = mb[c]; S.km_hw[0][c] = hw[c]; S.km_cplx[0][c] = c > 0; }

@@ -1,0 +1,17 @@
+"""The reciprocal / square-root helpers of the hot paths (rcp_nr, sqrt_nr in
+csrc/pwa_math.hpp) agree with IEEE 1/x and sqrt(x) on the GPU to <= 1 ulp."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "cpp", "bin", "fastmath_check")
+
+
+@pytest.mark.gpu
+def test_rcp_sqrt_helpers_match_ieee():
+    assert os.path.exists(EXE), "build the library first (pwa-xib2jpsilambdak_amd/build.sh)"
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=60)
+    print(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
