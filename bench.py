@@ -75,11 +75,14 @@ def work_model(model):
     ncls = len({(d.spin2[r], d.parity[r]) for r in range(d.n_lambda, d.n_lambda + d.n_pc)
                 if d.type[r] == abi.PWA_KMATRIX})
 
-    def flops(stage, n_events, nbins, smeared):
-        per = {"lam": 64.0 * k_lx, "pcang": 64.0 * k_pc + 64.0 * R * R,
-               "shape": (8.0 * nbins * R * R + 299.0 * nbins * ncls) if smeared else 0.0,
+    def flops(stage, n_events, nbins, n_smeared):
+        """Work of one launch over n_events events of which n_smeared are smeared."""
+        per = {"lam": 64.0 * k_lx, "pcang": 64.0 * k_pc + 64.0 * R * R, "shape": 0.0,
                "adj_lam": 128.0 * k_lx, "adj_pc": 128.0 * k_pc}[stage]
-        return per * n_events
+        f = per * n_events
+        if stage == "shape":
+            f += (8.0 * nbins * R * R + 299.0 * nbins * ncls) * n_smeared
+        return f
 
     return K, flops
 
@@ -202,6 +205,7 @@ def main():
     #     context's stream, recorded around each kernel)
     K, flops = work_model(model)
     nb = model.desc.scheme_nbins[0]
+    n_smeared = {abi.PWA_DATA: int(np.sum(ds > 0)), abi.PWA_MC: int(np.sum(ms > 0))}
     tot = {st: [0.0, 0.0, 0] for st in STAGES}  # ms, flops, launches
     for step in launches:
         for L in step:
@@ -210,7 +214,7 @@ def main():
                 if t_ms <= 0.0:
                     continue
                 tot[st][0] += t_ms
-                tot[st][1] += flops(st, L["n"], nb, L["scheme"] > 0)
+                tot[st][1] += flops(st, L["n"], nb, n_smeared[L["kind"]])
                 tot[st][2] += 1
     dom = max(tot, key=lambda st: tot[st][0])
     ms_sum, fl_sum, nlaunch = tot[dom]
@@ -219,7 +223,7 @@ def main():
     share = {st: round(v[0] / args.steps, 4) for st, v in tot.items()}
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic_per_launch(dom, nd + nm, nlaunch // args.steps),
-            "kernel": f"k_{dom} (gradient pipeline; all data + MC segment launches)",
+            "kernel": f"k_{dom} (gradient pipeline; data and MC sample launches)",
             "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": fl_sum / nlaunch,
             "stage_ms_per_step": share, "K_couplings": K}
 

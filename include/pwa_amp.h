@@ -210,6 +210,28 @@ int pwa_eval(pwa_ctx* ctx, const double* params, double* nll, double* integral,
 int pwa_eval_grad(pwa_ctx* ctx, const double* params, double* nll, double* grad);
 
 /*
+ * Toy batches: uploads n_toys data sets as the data sample, toy t being
+ * events [toy_offsets[t], toy_offsets[t+1]) of the column arrays (same layout
+ * as pwa_upload_events, n = toy_offsets[n_toys]); all toys share the MC
+ * sample.  The reduction of every evaluation is then done per toy.  No
+ * reference counterpart: the reference evaluates one data set per
+ * RooDalitzAmplitude (its toy studies re-run the whole fit per toy,
+ * gen_toys/ pipeline).  pwa_eval / pwa_eval_grad treat the toys as one data
+ * set (their union).
+ */
+int pwa_upload_toys(pwa_ctx* ctx, int32_t n_toys, const int64_t* toy_offsets, const double* cols,
+                    int64_t ld, const double* weight, const int32_t* scheme);
+
+/*
+ * Batched evaluation: the NLL of every toy data set (one data set when the
+ * data were uploaded with pwa_upload_events) at n_points fit-parameter
+ * vectors params[p * n_params ...], and optionally its gradient:
+ *   nll[p * n_toys + t], grad[(p * n_toys + t) * n_params + k] (grad may be NULL).
+ * Each toy's NLL is formed as pwa_eval's with that toy's weight sum.
+ */
+int pwa_eval_batch(pwa_ctx* ctx, int32_t n_points, const double* params, double* nll, double* grad);
+
+/*
  * Fit-level objective, as Minuit sees it (src/fit.cpp:312-317):
  *   fcn = (nll - NLL_SHIFT) * alpha + sum_c penalty_c(x_{idx_c})
  * with the one-sided Gaussian penalty of src/RooFGaussian.cxx:60-75,
@@ -232,14 +254,16 @@ int pwa_amp2(pwa_ctx* ctx, int kind, const double* params, double* amp2);
    context's stream): all kernels, and the longest single kernel launch.   */
 int pwa_last_timing(pwa_ctx* ctx, double* ms_total, double* ms_main);
 
-/* Per-segment device times of the last evaluation (HIP events on the
-   context's stream).  Segment q is one run of the stage kernels over the
-   events of one sample and smearing scheme; ms[5q .. 5q+4] are the times of
-   its five kernels in launch order: Lambda* and Zcs amplitude (k_lam), Pc angular
-   sums (k_pcang), Pc lineshapes and likelihood term (k_shape), Lambda* and Zcs
-   adjoint (k_adj_lam), Pc adjoint (k_adj_pc); kernels that did not run read
-   0.  kind, scheme, is_grad and nevents describe the segment.  *n receives
-   the number of segments (up to cap are written; ms holds 5 cap values). */
+/* Per-sample device times of the last evaluation (HIP events on the
+   context's stream).  Launch q is the stage pipeline over one sample (data,
+   then MC); ms[5q .. 5q+4] are the times of its five kernels in launch
+   order: Lambda* and Zcs amplitude (k_lam, with k_lsgen when it runs), Pc
+   angular sums (k_pcang), Pc lineshapes and likelihood term (k_shape),
+   Lambda* and Zcs adjoint (k_adj_lam), Pc adjoint (k_adj_pc); kernels that
+   did not run read 0.  kind is the sample, scheme its smearing scheme when
+   it has one (else -1), is_grad whether the gradient kernels ran, nevents
+   its event count.  *n receives the number of launches (up to cap are
+   written; ms holds 5 cap values). */
 int pwa_last_launches(pwa_ctx* ctx, int32_t cap, int32_t* n, double* ms, int32_t* kind,
                       int32_t* scheme, int32_t* is_grad, int64_t* nevents);
 

@@ -87,11 +87,34 @@ struct Partial {
     double s1, c1;  // data: count of f <= 0     MC: sum w*f_noint
 };
 
-// Arguments of the per-segment stage pipeline (pwa_kernels.hip).
+// A run of events of one sample with one smearing scheme (and one toy data
+// set); the blocks [blk0, blk0 + nblk) of the reducing kernels (k_shape,
+// k_adj_*) stride over its events.
+struct SegDesc {
+    int64_t begin, n;  // event range, device order
+    int32_t scheme;    // smearing scheme (0 = unsmeared)
+    int32_t nblk, blk0;
+    int32_t pad;
+};
+
+// Block -> segment view of a reducing-kernel launch over a whole sample.
+struct SegView {
+    const SegDesc* segs;
+    const int32_t* blkmap;   // segment of each block
+    int32_t force_unsmeared; // evaluate every segment with scheme 0
+    __device__ __forceinline__ SegDesc seg() const {
+        SegDesc s = segs[blkmap[blockIdx.x]];
+        if (force_unsmeared) s.scheme = 0;
+        return s;
+    }
+};
+
+// Arguments of the stage pipeline over one sample (pwa_stages.hpp).
 struct StageArgs {
     const double* ev;
-    int64_t ld, n;
-    int scheme;
+    int64_t ld, n;        // sample planes and event count
+    SegView sv;
+    int nblocks;          // blocks of the reducing kernels (sum of the segments' nblk)
     bool mc, grad;
     bool full;            // K-matrix members or generic Pc lineshapes present (full k_shape variant)
     int noint;
@@ -99,19 +122,17 @@ struct StageArgs {
     const DevDyn* D;
     const C64* cpl;
     double* inter;        // inter_planes(nz) x ld intermediate planes
-    double* fx;           // per-event output or null
-    Partial* part;        // shape_blocks partial sums
-    int shape_blocks, adj_blocks;
+    double* fx;           // per-event output (device order) or null
+    Partial* part;        // nblocks partial sums
     int ncpl, nres, ng;   // gradient layout
-    double* gpart_shape;  // shape_blocks x ng (gradient)
-    double* gpart_adj;    // adj_blocks x ng (gradient, Lambda* / Zcs adjoint)
-    double* gpart_adj_pc; // adj_blocks x ng (gradient, Pc adjoint; NZ > 0)
+    double* gpart;        // gradient partial rows: block b of k_shape / k_adj_lam / k_adj_pc -> rows rpb b + 0 / 1 / 2
+    int rpb;              // gradient rows per block (2, or 3 with Pc-chain resonances)
     hipEvent_t* tev;      // null, or kStageEvents events recorded around the stage kernels
     double* gen;          // generic-lineshape buffer (6 ngen planes x ld), null when ngen = 0
     int ngen;             // Lambda* / Zcs resonances outside the factorised form
 };
 
-// Stage kernels of one segment, in launch order (timing slots).
+// Stage kernels of one sample, in launch order (timing slots).
 enum Stage { ST_LAM = 0, ST_PCANG, ST_SHAPE, ST_ADJ_LAM, ST_ADJ_PC, ST_COUNT };
 constexpr int kStageEvents = ST_COUNT + 1;
 
@@ -123,9 +144,11 @@ hipError_t run_stages(int nz, const StageArgs& a, hipStream_t st);
 template <int NZ>
 hipError_t run_stages_nz(const StageArgs& a, hipStream_t st);  // one instantiation per NZ = 0..8
 hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, double* out, hipStream_t st);
-hipError_t launch_grad_reduce(const double* gpart, int b0, int b1, int b2, int NG, double* scratch, double* out,
-                              hipStream_t st);
-size_t grad_reduce_scratch(int rows_data, int rows_mc, int NG);
+// Gradient rows of group g: [gbeg[g], gbeg[g+1]); chunks: ranges of <= kGradChunk rows inside one group,
+// chunk k = rows [chunk[2k], chunk[2k+1]); group g owns chunks [cbeg[g], cbeg[g+1]).
+constexpr int kGradChunk = 128;
+hipError_t launch_grad_reduce(const double* gpart, int nchunk, const int32_t* chunk, int ngroups, const int32_t* cbeg,
+                              int NG, double* scratch, double* out, hipStream_t st);
 size_t grad_lds_bytes(int NG);
 int inter_planes(int nz);
 constexpr int kBlock = 256;

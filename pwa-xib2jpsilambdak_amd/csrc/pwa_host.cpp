@@ -29,10 +29,25 @@ struct Sample {
     double* gen = nullptr;              // generic-lineshape planes (grown on demand)
     int64_t gen_planes = 0;
     std::vector<int64_t> perm;          // device position -> caller index
-    std::vector<int> seg_scheme;
-    std::vector<int64_t> seg_begin, seg_count;
+    std::vector<SegDesc> segs;          // (toy, scheme) runs in device order
+    SegDesc* dsegs = nullptr;           // device copy
+    int32_t* dblkmap = nullptr;         // reduction block -> segment
+    int nblocks = 0;                    // reduction blocks over the sample
+    int n_toys = 1;                     // data sets in the sample (toy batches)
+    std::vector<int32_t> toy_blk;       // first reduction block of each toy, then nblocks
+    std::vector<double> toy_sumw;       // per toy: sum of weights
     double sumw = 0.0, sumw2 = 0.0;     // raw weights (data: sw; MC: w)
 };
+
+void free_sample(Sample& s) {
+    if (s.ev) (void)hipFree(s.ev);
+    if (s.fx) (void)hipFree(s.fx);
+    if (s.inter) (void)hipFree(s.inter);
+    if (s.gen) (void)hipFree(s.gen);
+    if (s.dsegs) (void)hipFree(s.dsegs);
+    if (s.dblkmap) (void)hipFree(s.dblkmap);
+    s = Sample();
+}
 
 } // namespace
 
@@ -52,25 +67,32 @@ struct pwa_ctx {
     Sample smp[2];
     Partial* part = nullptr;
     int64_t part_cap = 0;
-    int32_t* dbeg = nullptr;
-    double* dout = nullptr;
-    double* gpart = nullptr;  // per-block partial gradients
+    double* gpart = nullptr;  // per-block partial gradients, then the chunk sums of the reduction
     int64_t gpart_cap = 0;
-    double* dgout = nullptr;  // reduced gradients [2][NG]
-    double* hgout = nullptr;  // pinned copy
     int NG = 0;
-    double* hout = nullptr;   // pinned
+    // reduction groups of the last evaluation: the data sets (toys) of the
+    // data sample, then the MC sample; per group 4 sums (s0, c0, s1, c1) and
+    // for the gradient one NG-vector
+    int ngroups = 0;
+    int64_t grp_cap = 0;
+    double* dout = nullptr;   // [grp_cap][4]
+    double* hout = nullptr;   // pinned copy
+    double* dgout = nullptr;  // [grp_cap][NG]
+    double* hgout = nullptr;  // pinned copy
+    int32_t* didx = nullptr;  // reduction index tables (device)
+    int32_t* hidx = nullptr;  // pinned staging
+    int64_t idx_cap = 0;
     DevDyn* hdyn = nullptr;   // pinned
     C64* hcpl = nullptr;      // pinned
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float last_ms = 0.f;
     // per-launch timing of the last evaluation
     static constexpr int kMaxLaunch = 32;
-    hipEvent_t le[kMaxLaunch][kStageEvents] = {};  // per segment: around each stage kernel
+    hipEvent_t le[kMaxLaunch][kStageEvents] = {};  // per sample launch: around each stage kernel
     int nlaunch = 0;
     int32_t l_kind[kMaxLaunch] = {}, l_scheme[kMaxLaunch] = {}, l_grad[kMaxLaunch] = {};
     int64_t l_n[kMaxLaunch] = {};
-    float l_ms[kMaxLaunch] = {};         // whole segment
+    float l_ms[kMaxLaunch] = {};         // whole sample pipeline
     float l_stage[kMaxLaunch][ST_COUNT] = {};  // per stage kernel (enum Stage)
     std::string err;
     std::vector<int> cidx;
@@ -288,38 +310,84 @@ void collect_times(pwa_ctx* c) {
     }
 }
 
-constexpr int kMaxShapeBlocks = 2048;
-constexpr int kMaxAdjBlocks = 1024;
+// Grows the per-group result buffers to G groups.
+int ensure_groups(pwa_ctx* c, int G) {
+    if (G <= c->grp_cap) return PWA_OK;
+    if (c->dout) (void)hipFree(c->dout);
+    if (c->hout) (void)hipHostFree(c->hout);
+    if (c->dgout) (void)hipFree(c->dgout);
+    if (c->hgout) (void)hipHostFree(c->hgout);
+    c->dout = c->hout = c->dgout = c->hgout = nullptr;
+    c->grp_cap = 0;
+    HIPCHK(c, hipMalloc(&c->dout, (size_t)4 * G * sizeof(double)));
+    HIPCHK(c, hipHostMalloc(&c->hout, (size_t)4 * G * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->dgout, (size_t)G * c->NG * sizeof(double)));
+    HIPCHK(c, hipHostMalloc(&c->hgout, (size_t)G * c->NG * sizeof(double)));
+    c->grp_cap = G;
+    return PWA_OK;
+}
 
-// Runs the stage pipeline (k_lam, k_pcang, k_shape[, k_adj_lam, k_adj_pc]) on every segment of both
-// samples, then the reductions; leaves the sums in c->hout (data: s0,c0,s1,c1;
-// MC: s0,c0,s1,c1) and, for the gradient, G_data and G_mc in c->hgout.
+// Runs the stage pipeline (k_lsgen, k_lam, k_pcang, k_shape[, k_adj_lam,
+// k_adj_pc]) over each sample, then the reductions; leaves per group
+// (data toys 0..T-1, MC = group T) the sums (s0, c0, s1, c1) in c->hout and,
+// for the gradient, the group's gradient vector in c->hgout.
 int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool force_unsmeared, bool mc_as_data) {
     int rc = upload_params(c, params);
     if (rc) return rc;
     const int NG = c->NG;
-    struct Seg { int k; size_t q; int sb, ab; };
-    const int n_adj = c->desc.n_pc > 0 ? 2 : 1;  // adjoint kernels per segment
-    std::vector<Seg> segs;
-    int64_t prow = 0, grow = 0;
-    int32_t beg[6] = {0, 0, 0, 0, 0, 0};
+    const int rpb = c->desc.n_pc > 0 ? 3 : 2;  // gradient rows per block: k_shape, k_adj_lam[, k_adj_pc]
+    const Sample& sd = c->smp[PWA_DATA];
+    const Sample& sm = c->smp[PWA_MC];
+    const int T = sd.present ? sd.n_toys : 1;
+    const int G = T + 1;
+    if ((rc = ensure_groups(c, G))) return rc;
+    c->ngroups = G;
+    int64_t P0[2] = {0, 0}, R0[2] = {0, 0}, prow = 0, grow = 0;
     for (int k = 0; k < 2; ++k) {
-        beg[k] = (int32_t)prow;
-        beg[3 + k] = (int32_t)grow;
-        const Sample& s = c->smp[k];
-        if (!s.present) continue;
-        for (size_t q = 0; q < s.seg_scheme.size(); ++q) {
-            const int64_t n = s.seg_count[q];
-            if (n == 0) continue;
-            const int sb = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxShapeBlocks);
-            const int ab = grad ? (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxAdjBlocks) : 0;
-            segs.push_back({k, q, sb, ab});
-            prow += sb;
-            grow += grad ? sb + ab * n_adj : 0;
-        }
+        P0[k] = prow;
+        R0[k] = grow;
+        if (!c->smp[k].present) continue;
+        prow += c->smp[k].nblocks;
+        grow += grad ? (int64_t)rpb * c->smp[k].nblocks : 0;
     }
-    beg[2] = (int32_t)prow;
-    beg[5] = (int32_t)grow;
+    // reduction tables: part-row boundaries of the groups, gradient chunks
+    std::vector<int32_t> pbeg(G + 1), cbeg(G + 1), chunk;
+    for (int g = 0; g < G; ++g) {
+        int64_t blo = 0, bhi = 0, base = 0;
+        if (g < T) {
+            if (sd.present) { blo = sd.toy_blk[g]; bhi = sd.toy_blk[g + 1]; }
+            base = 0;
+        } else if (sm.present) {
+            blo = 0;
+            bhi = sm.nblocks;
+            base = 1;
+        } else {
+            base = 1;
+        }
+        pbeg[g] = (int32_t)(P0[base] + blo);
+        cbeg[g] = (int32_t)(chunk.size() / 2);
+        if (grad)
+            for (int64_t r = R0[base] + rpb * blo; r < R0[base] + rpb * bhi; r += kGradChunk) {
+                chunk.push_back((int32_t)r);
+                chunk.push_back((int32_t)std::min<int64_t>(r + kGradChunk, R0[base] + rpb * bhi));
+            }
+    }
+    pbeg[G] = (int32_t)prow;
+    cbeg[G] = (int32_t)(chunk.size() / 2);
+    const int nchunk = (int)(chunk.size() / 2);
+    const int64_t nidx = 2 * (G + 1) + (int64_t)chunk.size();
+    if (nidx > c->idx_cap) {
+        if (c->didx) (void)hipFree(c->didx);
+        if (c->hidx) (void)hipHostFree(c->hidx);
+        c->didx = c->hidx = nullptr;
+        c->idx_cap = 0;
+        HIPCHK(c, hipMalloc(&c->didx, nidx * sizeof(int32_t)));
+        HIPCHK(c, hipHostMalloc(&c->hidx, nidx * sizeof(int32_t)));
+        c->idx_cap = nidx;
+    }
+    std::copy(pbeg.begin(), pbeg.end(), c->hidx);
+    std::copy(cbeg.begin(), cbeg.end(), c->hidx + (G + 1));
+    std::copy(chunk.begin(), chunk.end(), c->hidx + 2 * (G + 1));
     prow = std::max<int64_t>(prow, 1);
     if (prow > c->part_cap) {
         if (c->part) (void)hipFree(c->part);
@@ -327,8 +395,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         HIPCHK(c, hipMalloc(&c->part, prow * sizeof(Partial)));
         c->part_cap = prow;
     }
-    // gradient partial rows, then the chunk sums of the two-pass reduction
-    const int64_t gneed = grow * NG + (int64_t)(grad_reduce_scratch(beg[4] - beg[3], beg[5] - beg[4], NG) / sizeof(double));
+    const int64_t gneed = grow * NG + (int64_t)nchunk * NG;
     if (grad && gneed > c->gpart_cap) {
         if (c->gpart) (void)hipFree(c->gpart);
         c->gpart = nullptr;
@@ -338,53 +405,50 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
     const int noint = (c->desc.flags & PWA_F_NOINT_PENALTY) ? 1 : 0;
     c->nlaunch = 0;
     HIPCHK(c, hipEventRecord(c->e0, c->st));
-    int64_t pr = 0, gr = 0;
-    for (const Seg& g : segs) {
-        Sample& s = c->smp[g.k];
-        const int64_t off = s.seg_begin[g.q], n = s.seg_count[g.q];
+    HIPCHK(c, hipMemcpyAsync(c->didx, c->hidx, nidx * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    for (int k = 0; k < 2; ++k) {
+        Sample& s = c->smp[k];
+        if (!s.present || s.n == 0) continue;
         StageArgs a{};
-        a.ev = s.ev + off;
+        a.ev = s.ev;
         a.ld = s.ld;
-        a.n = n;
-        a.scheme = force_unsmeared ? 0 : s.seg_scheme[g.q];
-        a.mc = (g.k == PWA_MC) && !mc_as_data;
+        a.n = s.n;
+        a.sv = SegView{s.dsegs, s.dblkmap, force_unsmeared ? 1 : 0};
+        a.nblocks = s.nblocks;
+        a.mc = (k == PWA_MC) && !mc_as_data;
         a.grad = grad;
         a.full = c->hs.km_nclass > 0 || c->pc_generic;
         a.noint = noint;
         a.S = c->dS;
         a.D = c->dD;
         a.cpl = c->dC;
-        a.inter = s.inter + off;
-        a.gen = s.gen ? s.gen + off : nullptr;
+        a.inter = s.inter;
+        a.gen = s.gen;
         a.ngen = c->ngen;
-        a.fx = want_fx ? s.fx + off : nullptr;
-        a.part = c->part + pr;
-        a.shape_blocks = g.sb;
-        a.adj_blocks = g.ab;
+        a.fx = want_fx ? s.fx : nullptr;
+        a.part = c->part + P0[k];
         a.ncpl = c->ncpl;
         a.nres = c->nres;
         a.ng = NG;
-        a.gpart_shape = grad ? c->gpart + gr * NG : nullptr;
-        a.gpart_adj = grad ? c->gpart + (gr + g.sb) * NG : nullptr;
-        a.gpart_adj_pc = grad && n_adj > 1 ? c->gpart + (gr + g.sb + g.ab) * NG : nullptr;
+        a.gpart = grad ? c->gpart + R0[k] * NG : nullptr;
+        a.rpb = rpb;
         const int li = c->nlaunch < pwa_ctx::kMaxLaunch ? c->nlaunch++ : -1;
         if (li >= 0) {
             a.tev = c->le[li];
-            c->l_kind[li] = g.k;
-            c->l_scheme[li] = a.scheme;
-            c->l_n[li] = n;
+            c->l_kind[li] = k;
+            c->l_scheme[li] = s.segs.size() == 1 ? s.segs[0].scheme : -1;
+            c->l_n[li] = s.n;
             c->l_grad[li] = grad ? 1 : 0;
         }
         HIPCHK(c, run_stages(c->desc.n_pc, a, c->st));
-        pr += g.sb;
-        gr += grad ? g.sb + g.ab * n_adj : 0;
     }
-    HIPCHK(c, hipMemcpyAsync(c->dbeg, beg, sizeof(beg), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, launch_reduce(c->part, 2, c->dbeg, c->dout, c->st));
-    if (grad) HIPCHK(c, launch_grad_reduce(c->gpart, beg[3], beg[4], beg[5], NG, c->gpart + grow * NG, c->dgout, c->st));
+    HIPCHK(c, launch_reduce(c->part, G, c->didx, c->dout, c->st));
+    if (grad)
+        HIPCHK(c, launch_grad_reduce(c->gpart, nchunk, c->didx + 2 * (G + 1), G, c->didx + (G + 1), NG,
+                                     c->gpart + grow * NG, c->dgout, c->st));
     HIPCHK(c, hipEventRecord(c->e1, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 8 * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    if (grad) HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, 2 * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->hout, c->dout, 4 * G * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (grad) HIPCHK(c, hipMemcpyAsync(c->hgout, c->dgout, (size_t)G * NG * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     collect_times(c);
     return PWA_OK;
@@ -403,19 +467,170 @@ int copy_fx(pwa_ctx* c, int kind, double* out) {
     return PWA_OK;
 }
 
-double nll_from_sums(const pwa_ctx* c, double* integral) {
-    const double* o = c->hout;
-    const double data_sum = o[0] + o[1];
-    const double I = o[4] + o[5];
-    const double Inoint = o[6] + o[7];
+constexpr int kMaxSampleBlocks = 4096;  // reduction blocks per sample (partial rows)
+constexpr int kMaxToys = 65000;         // reduction groups per launch (grid y limit)
+
+// Uploads one sample: events reordered by (toy, smearing scheme) into
+// segments, reduction blocks spread over the segments in proportion to their
+// size (each segment gets at least one), event planes built by k_prep.
+int upload_sample(pwa_ctx* c, int kind, int64_t n, const double* cols, int64_t ld, const double* weight,
+                  const int32_t* scheme, int n_toys, const int64_t* toy_off) {
+    if (n < 0 || (n > 0 && (!cols || !weight || ld < n))) return fail(c, PWA_E_ARG, "bad event arrays");
+    if (n > (int64_t)1 << 31) return fail(c, PWA_E_ARG, "too many events");
+    for (int64_t i = 0; scheme && i < n; ++i)
+        if (scheme[i] < 0 || scheme[i] > c->desc.n_schemes)
+            return fail(c, PWA_E_ARG, "scheme id out of range at event " + std::to_string(i));
+    HIPCHK(c, hipSetDevice(c->device));
+    Sample& s = c->smp[kind];
+    free_sample(s);
+    s.n = n;
+    s.ld = std::max<int64_t>(((n + 255) / 256) * 256, 256);
+    s.n_toys = n_toys;
+    std::vector<int32_t> toy(n, 0);
+    if (toy_off)
+        for (int t = 0; t < n_toys; ++t)
+            for (int64_t i = toy_off[t]; i < toy_off[t + 1]; ++i) toy[i] = t;
+    // order by (toy, scheme), stable: every segment shares one toy and one smearing scheme
+    s.perm.resize(n);
+    std::iota(s.perm.begin(), s.perm.end(), 0);
+    std::stable_sort(s.perm.begin(), s.perm.end(), [&](int64_t a, int64_t b) {
+        const int sa = scheme ? scheme[a] : 0, sb = scheme ? scheme[b] : 0;
+        return toy[a] != toy[b] ? toy[a] < toy[b] : sa < sb;
+    });
+    std::vector<int> seg_toy;
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t i = s.perm[q];
+        const int sc = scheme ? scheme[i] : 0;
+        if (s.segs.empty() || s.segs.back().scheme != sc || seg_toy.back() != toy[i]) {
+            s.segs.push_back(SegDesc{q, 0, sc, 0, 0, 0});
+            seg_toy.push_back(toy[i]);
+        }
+        s.segs.back().n++;
+    }
+    int64_t want = 0;
+    for (const SegDesc& g : s.segs) want += (g.n + kBlock - 1) / kBlock;
+    std::vector<int32_t> blkmap;
+    for (size_t q = 0; q < s.segs.size(); ++q) {
+        SegDesc& g = s.segs[q];
+        const int64_t full = (g.n + kBlock - 1) / kBlock;
+        g.nblk = (int32_t)(want <= kMaxSampleBlocks ? full
+                                                    : std::max<int64_t>(1, (int64_t)kMaxSampleBlocks * g.n / n));
+        g.blk0 = (int32_t)blkmap.size();
+        blkmap.insert(blkmap.end(), g.nblk, (int32_t)q);
+    }
+    s.nblocks = (int)blkmap.size();
+    s.toy_blk.assign(n_toys + 1, 0);
+    {
+        size_t q = 0;
+        for (int t = 0; t < n_toys; ++t) {
+            s.toy_blk[t] = q < s.segs.size() ? s.segs[q].blk0 : s.nblocks;
+            while (q < s.segs.size() && seg_toy[q] == t) ++q;
+        }
+        s.toy_blk[n_toys] = s.nblocks;
+    }
+    s.toy_sumw.assign(n_toys, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        s.sumw += weight[i];
+        s.sumw2 += weight[i] * weight[i];
+        s.toy_sumw[toy[i]] += weight[i];
+    }
+    // raw columns in device order (PWA_NCOL columns + weight)
+    const int ncol = PWA_NCOL + 1;
+    std::vector<double> raw((size_t)ncol * std::max<int64_t>(n, 1));
+    for (int col = 0; col < PWA_NCOL; ++col)
+        for (int64_t q = 0; q < n; ++q) raw[(size_t)col * n + q] = cols[(size_t)col * ld + s.perm[q]];
+    for (int64_t q = 0; q < n; ++q) raw[(size_t)PWA_NCOL * n + q] = weight[s.perm[q]];
+    double* draw = nullptr;
+    HIPCHK(c, hipMalloc(&s.ev, (size_t)E_NPLANE * s.ld * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.fx, (size_t)s.ld * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.inter, (size_t)inter_planes(c->desc.n_pc) * s.ld * sizeof(double)));
+    HIPCHK(c, hipMalloc(&s.dsegs, std::max<size_t>(s.segs.size(), 1) * sizeof(SegDesc)));
+    HIPCHK(c, hipMalloc(&s.dblkmap, std::max<size_t>(blkmap.size(), 1) * sizeof(int32_t)));
+    HIPCHK(c, hipMemsetAsync(s.ev, 0, (size_t)E_NPLANE * s.ld * sizeof(double), c->st));
+    if (!s.segs.empty()) {
+        HIPCHK(c, hipMemcpyAsync(s.dsegs, s.segs.data(), s.segs.size() * sizeof(SegDesc), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(s.dblkmap, blkmap.data(), blkmap.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    }
+    if (n > 0) {
+        HIPCHK(c, hipMalloc(&draw, raw.size() * sizeof(double)));
+        HIPCHK(c, hipMemcpyAsync(draw, raw.data(), raw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, launch_prep(draw, n, n, s.ev, s.ld, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (draw) (void)hipFree(draw);
+    s.present = true;
+    return PWA_OK;
+}
+
+// Compensated sum of the data sums of all groups (toys) of the last evaluation.
+double data_total(const pwa_ctx* c) {
+    double t = 0.0, e = 0.0;
+    for (int g = 0; g + 1 < c->ngroups; ++g) {
+        const double x = c->hout[4 * g] + c->hout[4 * g + 1];
+        const double u = t + x;
+        e += (std::fabs(t) >= std::fabs(x)) ? (t - u) + x : (x - u) + t;
+        t = u;
+    }
+    return t + e;
+}
+
+// NLL of a data set with weight sum W and data sum (sum sw ln f) from the
+// last evaluation's MC group: NLL_SHIFT + W ln(I / W_mc) - data + penalty
+// (host_store_fx, src/kernel_calEva.cu:1339-1361).
+double nll_assemble(const pwa_ctx* c, double W, double data_sum, double* integral) {
+    const double* o = c->hout + 4 * (c->ngroups - 1);
+    const double I = o[0] + o[1];
+    const double Inoint = o[2] + o[3];
     if (integral) *integral = I;
     const Sample& d = c->smp[PWA_DATA];
     const Sample& m = c->smp[PWA_MC];
     double v = c->desc.nll_shift;
-    if (m.present && d.present) v += d.sumw * std::log(I / m.sumw);
+    if (m.present && d.present) v += W * std::log(I / m.sumw);
     v -= data_sum;
     if ((c->desc.flags & PWA_F_NOINT_PENALTY) && std::fabs(Inoint / I - 1.0) > 0.3) v += 1000.0;
     return v;
+}
+
+double nll_from_sums(const pwa_ctx* c, double* integral) {
+    return nll_assemble(c, c->smp[PWA_DATA].sumw, data_total(c), integral);
+}
+
+// Kernel-level gradient gk = (W / I) G_mc - G_data mapped onto the
+// fit-parameter vector (helicity-coupling chain rule, masses, widths,
+// K-matrix blocks).
+void map_gradient(const pwa_ctx* c, const double* params, const std::vector<double>& gk, double* grad) {
+    const pwa_model_desc& d = c->desc;
+    int32_t np = 0;
+    pwa_fit_param_count(&d, &np);
+    std::fill(grad, grad + np, 0.0);
+    std::vector<cplx> gc(c->ncpl);
+    for (int j = 0; j < c->ncpl; ++j) gc[j] = cplx(gk[2 * j], gk[2 * j + 1]);
+    c->cmap.chain(params, gc.data(), grad);
+    const int o_res = 2 * c->ncpl;
+    const int o_km = o_res + 2 * c->nres;
+    int npole[4] = {0, 0, 0, 0}, nnr[4] = {0, 0, 0, 0};
+    for (int r = 0; r < c->nres; ++r) {
+        const int off = param_offset(d, r), size = param_size(d, r);
+        grad[off + size - 5] += gk[o_res + 2 * r];
+        if (d.type[r] != PWA_KMATRIX) {
+            grad[off + size - 4] += gk[o_res + 2 * r + 1];
+            continue;
+        }
+        const int ik = c->hs.ik[r];
+        const double* gkm = &gk[o_km + kGradNClassParams * ik];
+        if (params[off + size - 5] > 0) {
+            const int p = npole[ik]++;
+            if (p >= PWA_KM_NPOLE) continue;
+            for (int ch = 0; ch < PWA_KM_NCH; ++ch) grad[off + 8 + ch] += gkm[p * PWA_KM_NCH + ch];
+            grad[off + size - 5] += gkm[32 + p];
+        } else {
+            const int i = nnr[ik]++;
+            if (i >= PWA_KM_NCH) continue;
+            grad[off + 8] += gkm[16 + i];
+            if ((d.flags & PWA_F_KM_FULL_F) && i >= 1)
+                for (int j = i; j < PWA_KM_NCH; ++j) grad[off + 9 + (j - i)] += gkm[16 + i * PWA_KM_NCH + j];
+        }
+    }
 }
 
 } // namespace
@@ -455,12 +670,11 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipMalloc(&c->dS, sizeof(DevStatic)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dD, sizeof(DevDyn)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dC, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
-    if (bad(hipMalloc(&c->dbeg, 8 * sizeof(int32_t)))) return PWA_E_NOMEM;
-    if (bad(hipMalloc(&c->dout, 8 * sizeof(double)))) return PWA_E_NOMEM;
     c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams;
-    if (bad(hipMalloc(&c->dgout, 2 * c->NG * sizeof(double)))) return PWA_E_NOMEM;
-    if (bad(hipHostMalloc(&c->hgout, 2 * c->NG * sizeof(double)))) return PWA_E_NOMEM;
-    if (bad(hipHostMalloc(&c->hout, 8 * sizeof(double)))) return PWA_E_NOMEM;
+    if (ensure_groups(c, 2) != PWA_OK) {
+        pwa_destroy(c);
+        return PWA_E_NOMEM;
+    }
     if (bad(hipHostMalloc(&c->hdyn, sizeof(DevDyn)))) return PWA_E_NOMEM;
     if (bad(hipHostMalloc(&c->hcpl, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
     if (bad(hipEventCreate(&c->e0))) return PWA_E_HIP;
@@ -477,17 +691,13 @@ int pwa_destroy(pwa_ctx* c) {
     if (!c) return PWA_E_ARG;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    for (int k = 0; k < 2; ++k) {
-        if (c->smp[k].ev) (void)hipFree(c->smp[k].ev);
-        if (c->smp[k].fx) (void)hipFree(c->smp[k].fx);
-        if (c->smp[k].inter) (void)hipFree(c->smp[k].inter);
-        if (c->smp[k].gen) (void)hipFree(c->smp[k].gen);
-    }
+    for (int k = 0; k < 2; ++k) free_sample(c->smp[k]);
     if (c->dS) (void)hipFree(c->dS);
     if (c->dD) (void)hipFree(c->dD);
     if (c->dC) (void)hipFree(c->dC);
     if (c->part) (void)hipFree(c->part);
-    if (c->dbeg) (void)hipFree(c->dbeg);
+    if (c->didx) (void)hipFree(c->didx);
+    if (c->hidx) (void)hipHostFree(c->hidx);
     if (c->dout) (void)hipFree(c->dout);
     if (c->gpart) (void)hipFree(c->gpart);
     if (c->dgout) (void)hipFree(c->dgout);
@@ -511,58 +721,17 @@ int pwa_upload_events(pwa_ctx* c, int kind, int64_t n, const double* cols, int64
                       const int32_t* scheme) {
     if (!c) return PWA_E_ARG;
     if (kind != PWA_DATA && kind != PWA_MC) return fail(c, PWA_E_ARG, "kind must be PWA_DATA or PWA_MC");
-    if (n < 0 || (n > 0 && (!cols || !weight || ld < n))) return fail(c, PWA_E_ARG, "bad event arrays");
-    if (n > (int64_t)1 << 31) return fail(c, PWA_E_ARG, "too many events");
-    for (int64_t i = 0; scheme && i < n; ++i)
-        if (scheme[i] < 0 || scheme[i] > c->desc.n_schemes)
-            return fail(c, PWA_E_ARG, "scheme id out of range at event " + std::to_string(i));
-    HIPCHK(c, hipSetDevice(c->device));
-    Sample& s = c->smp[kind];
-    if (s.ev) (void)hipFree(s.ev);
-    if (s.fx) (void)hipFree(s.fx);
-    if (s.inter) (void)hipFree(s.inter);
-    if (s.gen) (void)hipFree(s.gen);
-    s = Sample();
-    s.n = n;
-    s.ld = std::max<int64_t>(((n + 255) / 256) * 256, 256);
-    // order by scheme (stable): every segment shares one smearing scheme
-    s.perm.resize(n);
-    std::iota(s.perm.begin(), s.perm.end(), 0);
-    if (scheme)
-        std::stable_sort(s.perm.begin(), s.perm.end(), [&](int64_t a, int64_t b) { return scheme[a] < scheme[b]; });
-    for (int64_t q = 0; q < n; ++q) {
-        const int sc = scheme ? scheme[s.perm[q]] : 0;
-        if (s.seg_scheme.empty() || s.seg_scheme.back() != sc) {
-            s.seg_scheme.push_back(sc);
-            s.seg_begin.push_back(q);
-            s.seg_count.push_back(0);
-        }
-        s.seg_count.back()++;
-    }
-    for (int64_t i = 0; i < n; ++i) {
-        s.sumw += weight[i];
-        s.sumw2 += weight[i] * weight[i];
-    }
-    // raw columns in device order (PWA_NCOL columns + weight)
-    const int ncol = PWA_NCOL + 1;
-    std::vector<double> raw((size_t)ncol * std::max<int64_t>(n, 1));
-    for (int col = 0; col < PWA_NCOL; ++col)
-        for (int64_t q = 0; q < n; ++q) raw[(size_t)col * n + q] = cols[(size_t)col * ld + s.perm[q]];
-    for (int64_t q = 0; q < n; ++q) raw[(size_t)PWA_NCOL * n + q] = weight[s.perm[q]];
-    double* draw = nullptr;
-    HIPCHK(c, hipMalloc(&s.ev, (size_t)E_NPLANE * s.ld * sizeof(double)));
-    HIPCHK(c, hipMalloc(&s.fx, (size_t)s.ld * sizeof(double)));
-    HIPCHK(c, hipMalloc(&s.inter, (size_t)inter_planes(c->desc.n_pc) * s.ld * sizeof(double)));
-    HIPCHK(c, hipMemsetAsync(s.ev, 0, (size_t)E_NPLANE * s.ld * sizeof(double), c->st));
-    if (n > 0) {
-        HIPCHK(c, hipMalloc(&draw, raw.size() * sizeof(double)));
-        HIPCHK(c, hipMemcpyAsync(draw, raw.data(), raw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, launch_prep(draw, n, n, s.ev, s.ld, c->st));
-    }
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    if (draw) (void)hipFree(draw);
-    s.present = true;
-    return PWA_OK;
+    return upload_sample(c, kind, n, cols, ld, weight, scheme, 1, nullptr);
+}
+
+int pwa_upload_toys(pwa_ctx* c, int32_t n_toys, const int64_t* toy_offsets, const double* cols, int64_t ld,
+                    const double* weight, const int32_t* scheme) {
+    if (!c || n_toys < 1 || !toy_offsets) return PWA_E_ARG;
+    if (n_toys > kMaxToys) return fail(c, PWA_E_ARG, "too many toy data sets");
+    if (toy_offsets[0] != 0) return fail(c, PWA_E_ARG, "toy_offsets[0] must be 0");
+    for (int t = 0; t < n_toys; ++t)
+        if (toy_offsets[t + 1] < toy_offsets[t]) return fail(c, PWA_E_ARG, "toy_offsets must be non-decreasing");
+    return upload_sample(c, PWA_DATA, toy_offsets[n_toys], cols, ld, weight, scheme, n_toys, toy_offsets);
 }
 
 int pwa_eval(pwa_ctx* c, const double* params, double* nll, double* integral, double* fx_data, double* fx_mc) {
@@ -619,47 +788,47 @@ int pwa_eval_grad(pwa_ctx* c, const double* params, double* nll, double* grad) {
         return fail(c, PWA_E_MODEL, "the analytic gradient supports one K-matrix J^P class per model");
     int rc = launch_all(c, params, true, false, false, false);
     if (rc) return rc;
-    const int NG = c->NG;
+    const int NG = c->NG, T = c->ngroups - 1;
     double I = 0.0;
     const double v = nll_from_sums(c, &I);
     if (nll) *nll = v;
-    // kernel-level gradient: sumW_data * G_mc / I - G_data
-    const double* Gd = c->hgout;
-    const double* Gm = c->hgout + NG;
+    // kernel-level gradient: sumW_data * G_mc / I - G_data (G_data summed over the data sets)
+    const double* Gm = c->hgout + (size_t)T * NG;
     const double sc = c->smp[PWA_DATA].sumw / I;
     std::vector<double> gk(NG);
-    for (int q = 0; q < NG; ++q) gk[q] = sc * Gm[q] - Gd[q];
-    // map to the fit-parameter vector
-    const pwa_model_desc& d = c->desc;
+    for (int q = 0; q < NG; ++q) {
+        double gd = 0.0;
+        for (int t = 0; t < T; ++t) gd += c->hgout[(size_t)t * NG + q];
+        gk[q] = sc * Gm[q] - gd;
+    }
+    map_gradient(c, params, gk, grad);
+    return PWA_OK;
+}
+
+int pwa_eval_batch(pwa_ctx* c, int32_t n_points, const double* params, double* nll, double* grad) {
+    if (!c || n_points < 0 || (n_points > 0 && (!params || !nll))) return PWA_E_ARG;
+    if (!c->smp[PWA_DATA].present || !c->smp[PWA_MC].present)
+        return fail(c, PWA_E_STATE, "both data and MC must be uploaded before pwa_eval_batch");
+    if (grad && c->hs.km_nclass > 1)
+        return fail(c, PWA_E_MODEL, "the analytic gradient supports one K-matrix J^P class per model");
     int32_t np = 0;
-    pwa_fit_param_count(&d, &np);
-    std::fill(grad, grad + np, 0.0);
-    std::vector<cplx> gc(c->ncpl);
-    for (int j = 0; j < c->ncpl; ++j) gc[j] = cplx(gk[2 * j], gk[2 * j + 1]);
-    c->cmap.chain(params, gc.data(), grad);
-    const int o_res = 2 * c->ncpl;
-    const int o_km = o_res + 2 * c->nres;
-    int npole[4] = {0, 0, 0, 0}, nnr[4] = {0, 0, 0, 0};
-    for (int r = 0; r < c->nres; ++r) {
-        const int off = param_offset(d, r), size = param_size(d, r);
-        grad[off + size - 5] += gk[o_res + 2 * r];
-        if (d.type[r] != PWA_KMATRIX) {
-            grad[off + size - 4] += gk[o_res + 2 * r + 1];
-            continue;
-        }
-        const int ik = c->hs.ik[r];
-        const double* gkm = &gk[o_km + kGradNClassParams * ik];
-        if (params[off + size - 5] > 0) {
-            const int p = npole[ik]++;
-            if (p >= PWA_KM_NPOLE) continue;
-            for (int ch = 0; ch < PWA_KM_NCH; ++ch) grad[off + 8 + ch] += gkm[p * PWA_KM_NCH + ch];
-            grad[off + size - 5] += gkm[32 + p];
-        } else {
-            const int i = nnr[ik]++;
-            if (i >= PWA_KM_NCH) continue;
-            grad[off + 8] += gkm[16 + i];
-            if ((d.flags & PWA_F_KM_FULL_F) && i >= 1)
-                for (int j = i; j < PWA_KM_NCH; ++j) grad[off + 9 + (j - i)] += gkm[16 + i * PWA_KM_NCH + j];
+    pwa_fit_param_count(&c->desc, &np);
+    const int T = c->smp[PWA_DATA].n_toys;
+    const int NG = c->NG;
+    std::vector<double> gk(NG);
+    for (int p = 0; p < n_points; ++p) {
+        const double* par = params + (size_t)p * np;
+        int rc = launch_all(c, par, grad != nullptr, false, false, false);
+        if (rc) return rc;
+        double I = 0.0;
+        for (int t = 0; t < T; ++t) {
+            const double W = c->smp[PWA_DATA].toy_sumw[t];
+            nll[(size_t)p * T + t] = nll_assemble(c, W, c->hout[4 * t] + c->hout[4 * t + 1], &I);
+            if (!grad) continue;
+            const double* Gm = c->hgout + (size_t)T * NG;
+            const double* Gd = c->hgout + (size_t)t * NG;
+            for (int q = 0; q < NG; ++q) gk[q] = (W / I) * Gm[q] - Gd[q];
+            map_gradient(c, par, gk, grad + ((size_t)p * T + t) * np);
         }
     }
     return PWA_OK;

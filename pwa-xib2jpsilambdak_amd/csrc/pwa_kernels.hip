@@ -129,24 +129,19 @@ hipError_t launch_reduce(const Partial* part, int ngroups, const int32_t* beg, d
     return hipGetLastError();
 }
 
-// Gradient partials: rows [b0, b1) belong to the data sample, [b1, b2) to
-// MC; out[k * NG + q] = sum of column q over the rows of sample k.  Two
-// deterministic passes: pass 1 sums chunks of kGradChunk rows (64 columns x 4
-// row-slices per block, slices combined in order), pass 2 sums the chunk
-// results of each sample in order.
-constexpr int kGradChunk = 128;
-
-__host__ __device__ inline int grad_chunks(int rows) { return (rows + kGradChunk - 1) / kGradChunk; }
-
-__global__ void __launch_bounds__(kBlock) k_grad_chunks(const double* __restrict__ gpart, int b0, int b1, int b2,
-                                                         int NG, double* __restrict__ chunk_out) {
+// Gradient partials: each group (a toy data set, or the MC sample) owns a
+// list of chunks of at most kGradChunk partial rows; out[g * NG + q] = sum of
+// column q over the rows of group g.  Two deterministic passes: pass 1 sums
+// each chunk (64 columns x 4 row-slices per block, slices combined in order),
+// pass 2 sums the chunk results of each group in order.
+__global__ void __launch_bounds__(kBlock) k_grad_chunks(const double* __restrict__ gpart,
+                                                         const int32_t* __restrict__ chunk, int NG,
+                                                         double* __restrict__ chunk_out) {
     __shared__ double red[4][64][2];
     const int col = blockIdx.x * 64 + (threadIdx.x & 63);
     const int slice = threadIdx.x >> 6;
     const int y = blockIdx.y;
-    const int n0 = grad_chunks(b1 - b0);
-    const int lo = (y < n0) ? b0 + y * kGradChunk : b1 + (y - n0) * kGradChunk;
-    const int hi = min(lo + kGradChunk, (y < n0) ? b1 : b2);
+    const int lo = chunk[2 * y], hi = chunk[2 * y + 1];
     double s = 0.0, c = 0.0;
     if (col < NG)
         for (int row = lo + slice; row < hi; row += 4) kadd(s, c, gpart[(int64_t)row * NG + col]);
@@ -163,31 +158,26 @@ __global__ void __launch_bounds__(kBlock) k_grad_chunks(const double* __restrict
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_grad_final(const double* __restrict__ chunk_out, int n0, int n1, int NG,
+__global__ void __launch_bounds__(kBlock) k_grad_final(const double* __restrict__ chunk_out,
+                                                        const int32_t* __restrict__ cbeg, int NG,
                                                         double* __restrict__ out) {
     const int q = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y;
+    const int g = blockIdx.y;
     if (q >= NG) return;
-    const int lo = k == 0 ? 0 : n0, hi = k == 0 ? n0 : n0 + n1;
     double t = 0.0, tc = 0.0;
-    for (int y = lo; y < hi; ++y) kadd(t, tc, chunk_out[(int64_t)y * NG + q]);
-    out[k * NG + q] = t + tc;
+    for (int y = cbeg[g]; y < cbeg[g + 1]; ++y) kadd(t, tc, chunk_out[(int64_t)y * NG + q]);
+    out[(int64_t)g * NG + q] = t + tc;
 }
 
-size_t grad_reduce_scratch(int rows_data, int rows_mc, int NG) {
-    return (size_t)(grad_chunks(rows_data) + grad_chunks(rows_mc)) * NG * sizeof(double);
-}
-
-hipError_t launch_grad_reduce(const double* gpart, int b0, int b1, int b2, int NG, double* scratch, double* out,
-                              hipStream_t st) {
-    const int n0 = grad_chunks(b1 - b0), n1 = grad_chunks(b2 - b1);
-    if (n0 + n1 > 0) {
-        hipLaunchKernelGGL(k_grad_chunks, dim3((NG + 63) / 64, n0 + n1), dim3(kBlock), 0, st, gpart, b0, b1, b2, NG,
-                           scratch);
+hipError_t launch_grad_reduce(const double* gpart, int nchunk, const int32_t* chunk, int ngroups, const int32_t* cbeg,
+                              int NG, double* scratch, double* out, hipStream_t st) {
+    if (nchunk > 0) {
+        hipLaunchKernelGGL(k_grad_chunks, dim3((NG + 63) / 64, nchunk), dim3(kBlock), 0, st, gpart, chunk, NG, scratch);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_grad_final, dim3((NG + kBlock - 1) / kBlock, 2), dim3(kBlock), 0, st, scratch, n0, n1, NG, out);
+    hipLaunchKernelGGL(k_grad_final, dim3((NG + kBlock - 1) / kBlock, ngroups), dim3(kBlock), 0, st, scratch, cbeg, NG,
+                       out);
     return hipGetLastError();
 }
 

@@ -63,17 +63,19 @@ __global__ void __launch_bounds__(kBlock, 3) k_lam(const double* __restrict__ ev
     }
 }
 
-__global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict__ ev, int64_t ld, int64_t n,
+__global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                    const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                    const C64* __restrict__ cpl, const double* __restrict__ gen,
                                                    const double* __restrict__ amp, const double* __restrict__ awp,
-                                                   GradLayout G, int NGp, double* __restrict__ gpart) {
+                                                   GradLayout G, int NGp, double* __restrict__ gpart, int rpb) {
     extern __shared__ double lds[];
     Emitter em = Emitter::make(lds, NGp);
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    const SegDesc sd = sv.seg();
+    const int64_t n = sd.n;
+    for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
         const int64_t gi = base + threadIdx.x;
         const bool valid = gi < n;
-        const int64_t i = valid ? gi : n - 1;
+        const int64_t i = sd.begin + (valid ? gi : n - 1);
         auto A = [&](int p) { return amp[(int64_t)p * ld + i]; };
         const double a2 = valid ? 2.0 * awp[i] : 0.0;
         {
@@ -91,7 +93,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict_
     }
     if (em.cnt > 0) em.flush();
     __syncthreads();
-    for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+    for (int q = threadIdx.x; q < G.ng; q += kBlock)
+        gpart[((int64_t)rpb * blockIdx.x + 1) * G.ng + q] = Emitter::block_sum(lds, NGp, q);
 }
 
 hipError_t launch_lam(const StageArgs& a, double* pcs, double* amp, bool store_h0, hipStream_t st) {
@@ -112,8 +115,8 @@ hipError_t launch_adj_lam(const StageArgs& a, const double* amp, const double* a
     if (e != hipSuccess) return e;
     GradLayout G{a.ncpl, a.nres, a.ng};
     const int NGp = (a.ng + 1) & ~1;
-    hipLaunchKernelGGL(k_adj_lam, dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n, a.S, a.D,
-                       a.cpl, a.gen, amp, awp, G, NGp, a.gpart_adj);
+    hipLaunchKernelGGL(k_adj_lam, dim3((unsigned)a.nblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S, a.D,
+                       a.cpl, a.gen, amp, awp, G, NGp, a.gpart, a.rpb);
     return hipGetLastError();
 }
 

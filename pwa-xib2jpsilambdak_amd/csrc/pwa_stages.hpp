@@ -71,22 +71,25 @@ __global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev,
 // FULL: K-matrix members or Pc lineshapes outside the factorised range are
 // present this evaluation (pc_lineshapes).
 template <int NZ, bool MC, bool GRAD, bool FULL>
-__global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev, int64_t ld, int64_t n, int scheme,
+__global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                  const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                  const double* __restrict__ pcs, double* __restrict__ amp,
                                                  double* __restrict__ awp, double* __restrict__ fx_out,
                                                  Partial* __restrict__ part, int noint, GradLayout G, int NGp,
-                                                 double* __restrict__ gpart) {
+                                                 double* __restrict__ gpart, int rpb) {
     extern __shared__ double lds[];
     Emitter em;
     if constexpr (GRAD) em = Emitter::make(lds, NGp);
     double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+    const SegDesc sd = sv.seg();
+    const int scheme = sd.scheme;
     const int nb = S->scheme_nbins[scheme];
     const double inb = 1.0 / nb;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    const int64_t n = sd.n;
+    for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
         const int64_t gi = base + threadIdx.x;
         const bool valid = gi < n;
-        const int64_t i = valid ? gi : n - 1;
+        const int64_t i = sd.begin + (valid ? gi : n - 1);
         auto P = [&](int p) { return pcs[(int64_t)p * ld + i]; };
         const double w = ld_d(ev, ld, i, E_W);
         const double mjp = ld_d(ev, ld, i, E_MJPSIP);
@@ -198,13 +201,13 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             v0 = unw ? f : f * w;
             v1 = (noint && !unw) ? fni * w : 0.0;
             aw = unw ? 1.0 : w;
-            if (valid && fx_out) fx_out[gi] = v0;
+            if (valid && fx_out) fx_out[i] = v0;
         } else {
             if (f > 0) v0 = log(f) * w;
             else v0 = (S->flags & PWA_F_NEG_FX_PENALTY) ? -1e12 : 0.0;
             v1 = (f > 0) ? 0.0 : 1.0;
             aw = (f > 0) ? w / f : 0.0;
-            if (valid && fx_out) fx_out[gi] = f;
+            if (valid && fx_out) fx_out[i] = f;
         }
         if (!valid) {
             v0 = 0.0;
@@ -285,7 +288,8 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
     if (threadIdx.x == 0) part[blockIdx.x] = Partial{s0, c0, s1, c1};
     if constexpr (GRAD) {
         __syncthreads();
-        for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+        for (int q = threadIdx.x; q < G.ng; q += kBlock)
+            gpart[((int64_t)rpb * blockIdx.x) * G.ng + q] = Emitter::block_sum(lds, NGp, q);
     }
 }
 
@@ -293,16 +297,18 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
 // Pc coupling adjoint from the seeds U_r that k_shape leaves in the amp
 // planes; every contribution weighted by 2 a.
 template <int NZ>
-__global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__ ev, int64_t ld, int64_t n,
+__global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                   const DevStatic* __restrict__ S, const double* __restrict__ amp,
                                                   const double* __restrict__ awp, GradLayout G, int NGp,
-                                                  double* __restrict__ gpart) {
+                                                  double* __restrict__ gpart, int rpb) {
     extern __shared__ double lds[];
     Emitter em = Emitter::make(lds, NGp);
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n; base += (int64_t)gridDim.x * kBlock) {
+    const SegDesc sd = sv.seg();
+    const int64_t n = sd.n;
+    for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
         const int64_t gi = base + threadIdx.x;
         const bool valid = gi < n;
-        const int64_t i = valid ? gi : n - 1;
+        const int64_t i = sd.begin + (valid ? gi : n - 1);
         auto A = [&](int p) { return amp[(int64_t)p * ld + i]; };
         const double a2 = valid ? 2.0 * awp[i] : 0.0;
         const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
@@ -318,7 +324,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__
     }
     if (em.cnt > 0) em.flush();
     __syncthreads();
-    for (int q = threadIdx.x; q < G.ng; q += kBlock) gpart[(int64_t)blockIdx.x * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+    for (int q = threadIdx.x; q < G.ng; q += kBlock)
+        gpart[((int64_t)rpb * blockIdx.x + 2) * G.ng + q] = Emitter::block_sum(lds, NGp, q);
 }
 
 template <int NZ>
@@ -341,7 +348,7 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if ((e = mark(2)) != hipSuccess) return e;
-    const unsigned sblocks = (unsigned)a.shape_blocks;
+    const unsigned sblocks = (unsigned)a.nblocks;
 #define PWA_SHAPE(MC, GR, FL)                                                                                      \
     do {                                                                                                           \
         if (GR) {                                                                                                  \
@@ -349,8 +356,8 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                         \
             if (e != hipSuccess) return e;                                                                         \
         }                                                                                                          \
-        hipLaunchKernelGGL((k_shape<NZ, MC, GR, FL>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,        \
-                           a.scheme, a.S, a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart_shape);            \
+        hipLaunchKernelGGL((k_shape<NZ, MC, GR, FL>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S,  \
+                           a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart, a.rpb);                      \
     } while (0)
     if (a.full) {
         if (a.mc && a.grad) PWA_SHAPE(true, true, true);
@@ -372,8 +379,8 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
         if constexpr (NZ > 0) {
             e = hipFuncSetAttribute((const void*)k_adj_pc<NZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_adj_pc<NZ>), dim3((unsigned)a.adj_blocks), dim3(kBlock), lds, st, a.ev, a.ld, a.n,
-                               a.S, amp, awp, G, NGp, a.gpart_adj_pc);
+            hipLaunchKernelGGL((k_adj_pc<NZ>), dim3((unsigned)a.nblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv,
+                               a.S, amp, awp, G, NGp, a.gpart, a.rpb);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         if ((e = mark(5)) != hipSuccess) return e;

@@ -40,6 +40,8 @@ def lib():
             "pwa_upload_events": (C.c_int, [vp, C.c_int, C.c_int64, _dp, C.c_int64, _dp, _ip]),
             "pwa_eval": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp]),
             "pwa_eval_grad": (C.c_int, [vp, _dp, _dp, _dp]),
+            "pwa_upload_toys": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64), _dp, C.c_int64, _dp, _ip]),
+            "pwa_eval_batch": (C.c_int, [vp, C.c_int32, _dp, _dp, _dp]),
             "pwa_set_constraints": (C.c_int, [vp, C.c_int32, _ip, _dp, _dp, _dp, _dp]),
             "pwa_fcn": (C.c_int, [vp, _dp, _dp, _dp]),
             "pwa_alpha": (C.c_int, [vp, _dp]),
@@ -122,6 +124,7 @@ class Context:
         check(lib().pwa_create(C.byref(desc), device, C.byref(self._h)), what="pwa_create")
         self.n_params = desc.n_params
         self.n = {abi.PWA_DATA: 0, abi.PWA_MC: 0}
+        self.n_toys = 1
 
     def close(self):
         if self._h:
@@ -152,6 +155,37 @@ class Context:
         check(lib().pwa_upload_events(self._h, kind, n, dptr(cols), n, dptr(w), iptr(sc)), self._h,
               "pwa_upload_events")
         self.n[kind] = n
+        if kind == abi.PWA_DATA:
+            self.n_toys = 1
+
+    def upload_toys(self, toy_sizes, cols, weight, scheme=None):
+        """Data sample made of len(toy_sizes) data sets, stored one after the
+        other in cols / weight / scheme (toy t: the next toy_sizes[t] events)."""
+        cols = np.ascontiguousarray(cols, dtype=np.float64)
+        if cols.shape[0] != abi.PWA_NCOL:
+            raise ValueError(f"cols must have {abi.PWA_NCOL} rows")
+        n = cols.shape[1]
+        off = np.concatenate([[0], np.cumsum(np.asarray(toy_sizes, dtype=np.int64))]).astype(np.int64)
+        if off[-1] != n:
+            raise ValueError("toy sizes must add up to the number of events")
+        w = np.ascontiguousarray(weight, dtype=np.float64)
+        sc = None if scheme is None else np.ascontiguousarray(scheme, dtype=np.int32)
+        check(lib().pwa_upload_toys(self._h, len(off) - 1, off.ctypes.data_as(C.POINTER(C.c_int64)), dptr(cols), n,
+                                    dptr(w), iptr(sc)), self._h, "pwa_upload_toys")
+        self.n[abi.PWA_DATA] = n
+        self.n_toys = len(off) - 1
+
+    def eval_batch(self, points, grad=False):
+        """NLL (and gradient) of every toy data set at every parameter point:
+        returns nll[n_points, n_toys] (and grad[n_points, n_toys, n_params])."""
+        pts = np.ascontiguousarray(np.atleast_2d(points), dtype=np.float64)
+        if pts.shape[1] != self.n_params:
+            raise ValueError(f"points must have {self.n_params} columns")
+        T = getattr(self, "n_toys", 1)
+        nll = np.zeros((pts.shape[0], T))
+        g = np.zeros((pts.shape[0], T, self.n_params)) if grad else None
+        check(lib().pwa_eval_batch(self._h, pts.shape[0], dptr(pts), dptr(nll), dptr(g)), self._h, "pwa_eval_batch")
+        return (nll, g) if grad else nll
 
     def eval(self, params, want_fx=False):
         p = np.ascontiguousarray(params, dtype=np.float64)

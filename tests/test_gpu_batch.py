@@ -1,0 +1,98 @@
+"""Toy batches and parameter-point batches (pwa_upload_toys / pwa_eval_batch)
+against single-data-set contexts and the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from pwa_amd import abi, models
+from pwa_amd._lib import Context, generate_phsp
+
+pytestmark = pytest.mark.gpu
+
+
+def _toys(model, sizes, seed):
+    cols = generate_phsp(int(sum(sizes)), seed)
+    rng = np.random.default_rng(seed)
+    sw = rng.uniform(0.5, 1.5, size=cols.shape[1])
+    sch = models.window_schemes(cols[abi.COL["mjpsiL"]]) if model.desc.n_schemes > 0 \
+        else np.zeros(cols.shape[1], np.int32)
+    return cols, sw, sch
+
+
+def _mc(model, nm, seed):
+    cols = generate_phsp(nm, seed)
+    w = np.random.default_rng(seed).uniform(0.8, 1.2, size=nm)
+    sch = models.window_schemes(cols[abi.COL["mjpsiL"]]) if model.desc.n_schemes > 0 else np.zeros(nm, np.int32)
+    return cols, w, sch
+
+
+def _points(model, n, seed):
+    rng = np.random.default_rng(seed)
+    pts = np.tile(model.params, (n, 1))
+    fl = np.flatnonzero(model.float_mask)
+    pts[1:, fl] *= 1.0 + 0.02 * rng.standard_normal((n - 1, len(fl)))
+    return pts
+
+
+@pytest.mark.parametrize("cfg", ["small", "nominal"])
+def test_toy_batch_matches_single_data_sets(cfg):
+    model = models.config_small(1) if cfg == "small" else models.config_nominal(2)
+    sizes = [700, 0, 1300, 45, 400]  # uneven, one empty data set
+    cols, sw, sch = _toys(model, sizes, 31)
+    mcols, mw, msch = _mc(model, 3000, 32)
+    pts = _points(model, 3, 33)
+    print(f"[batch] {model.name}: toys {sizes}, {pts.shape[0]} points", flush=True)
+    with Context(model.desc) as c:
+        c.upload_toys(sizes, cols, sw, sch)
+        c.upload(abi.PWA_MC, mcols, mw, msch)
+        nll, grad = c.eval_batch(pts, grad=True)
+        nll_only = c.eval_batch(pts)
+        # pwa_eval on a toy upload = the union of the toys as one data set
+        union_nll, _ = c.eval(pts[0])
+    assert np.array_equal(nll, nll_only)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    worst_nll = worst_g = 0.0
+    for t, n_t in enumerate(sizes):
+        sl = slice(off[t], off[t + 1])
+        with Context(model.desc) as c1:
+            c1.upload(abi.PWA_DATA, cols[:, sl], sw[sl], sch[sl])
+            c1.upload(abi.PWA_MC, mcols, mw, msch)
+            for p in range(pts.shape[0]):
+                v, g = c1.eval_grad(pts[p])
+                worst_nll = max(worst_nll, abs(nll[p, t] - v) / max(abs(v), 1.0))
+                worst_g = max(worst_g, np.max(np.abs(grad[p, t] - g)) / max(np.max(np.abs(g)), 1.0))
+    print(f"[batch] max rel NLL diff vs single sets {worst_nll:.2e}, max rel grad diff {worst_g:.2e}", flush=True)
+    assert worst_nll < 1e-12
+    assert worst_g < 1e-10
+    # the empty data set: NLL = NLL_SHIFT, zero data gradient
+    assert np.all(nll[:, 1] == model.desc.nll_shift)
+    with Context(model.desc) as cu:
+        cu.upload(abi.PWA_DATA, cols, sw, sch)
+        cu.upload(abi.PWA_MC, mcols, mw, msch)
+        v_union, _ = cu.eval(pts[0])
+    assert abs(union_nll - v_union) <= 1e-12 * abs(v_union)
+
+
+def test_toy_batch_against_oracle():
+    model = models.config_nominal(2)
+    sizes = [900, 250]
+    cols, sw, sch = _toys(model, sizes, 41)
+    mcols, mw, msch = _mc(model, 2000, 42)
+    with Context(model.desc) as c:
+        c.upload_toys(sizes, cols, sw, sch)
+        c.upload(abi.PWA_MC, mcols, mw, msch)
+        nll = c.eval_batch(model.params[None, :])
+    for t in range(2):
+        sl = slice(sum(sizes[:t]), sum(sizes[:t + 1]))
+        ref = oracle.nll(model.desc, model.params, cols[:, sl], sw[sl], sch[sl], mcols, mw, msch)
+        rel = abs(nll[0, t] - ref["nll"]) / abs(ref["nll"])
+        print(f"[batch] toy {t}: rel NLL vs oracle {rel:.2e}", flush=True)
+        assert rel < 1e-9
+
+
+def test_batch_rejects_bad_offsets():
+    model = models.config_small(1)
+    cols, sw, sch = _toys(model, [10, 10], 51)
+    with Context(model.desc) as c:
+        with pytest.raises(Exception):
+            c.upload_toys([10, 11], cols, sw, sch)
