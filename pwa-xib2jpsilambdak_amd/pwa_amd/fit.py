@@ -1,0 +1,293 @@
+"""Fit harness: a variable-metric minimiser over the floating parameters, the
+plain (finite-difference) and analytic gradient modes, random start points
+and multi-start fits.
+
+The reference drives Minuit's MIGRAD on the FCN of src/fit.cpp:312-317 with
+numerical derivatives (settings src/fit.cpp:240-261; random start smearing
+src/execute.cpp:194-284, RooTDalitz/Setup.h:323-474).  `minimize` is this
+project's own BFGS-type variable-metric method with MIGRAD's stopping rule
+(estimated distance to minimum EDM = g.V.g / 2 below a tolerance); the plain
+mode differentiates the forward FCN by central differences (2 P evaluations
+per gradient), the analytic mode uses the adjoint kernels (pwa_fcn with a
+gradient).
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+class Objective:
+    """FCN over the floating entries x of a full parameter vector.
+
+    mode = "analytic": one pwa_fcn call gives value and gradient.
+    mode = "fd"      : value from pwa_fcn, gradient by central differences
+                       with steps h_i = rel_step * max(|x_i|, 1e-2) (2P calls).
+    """
+
+    def __init__(self, ctx, template, float_mask, mode="analytic", rel_step=1e-5, record=False):
+        self.ctx = ctx
+        self.record = record
+        self.records = []   # (x, f, g) of every analytic evaluation when record
+        self.template = np.array(template, dtype=np.float64)
+        self.idx = np.flatnonzero(float_mask)
+        self.mode = mode
+        self.rel_step = rel_step
+        self.n_fcn = 0      # FCN evaluations (forward kernels)
+        self.n_grad = 0     # analytic gradient evaluations (adjoint kernels)
+
+    def full(self, x):
+        p = self.template.copy()
+        p[self.idx] = x
+        return p
+
+    def value(self, x):
+        self.n_fcn += 1
+        return self.ctx.fcn(self.full(x))
+
+    def value_grad(self, x):
+        if self.mode == "analytic":
+            self.n_grad += 1
+            f, g = self.ctx.fcn(self.full(x), grad=True)
+            g = g[self.idx]
+            if self.record:
+                self.records.append((np.array(x), f, g.copy()))
+            return f, g
+        f = self.value(x)
+        return f, self.fd_grad(x)
+
+    def fd_grad(self, x):
+        g = np.empty_like(x)
+        for k in range(len(x)):
+            h = self.rel_step * max(abs(x[k]), 1e-2)
+            xp = x.copy()
+            xm = x.copy()
+            xp[k] += h
+            xm[k] -= h
+            g[k] = (self.value(xp) - self.value(xm)) / (2.0 * h)
+        return g
+
+
+class Bounded:
+    """Box limits for an objective through MIGRAD's sine transform: internal
+    u, external x = lo + (hi - lo) (sin u + 1) / 2 for finite (lo, hi),
+    x = u otherwise.  Wraps an Objective; minimize() works on u."""
+
+    def __init__(self, obj, lo, hi):
+        self.obj, self.mode = obj, obj.mode
+        self.lo = np.asarray(lo, dtype=np.float64)
+        self.hi = np.asarray(hi, dtype=np.float64)
+        self.b = np.isfinite(self.lo) & np.isfinite(self.hi)
+
+    @property
+    def n_fcn(self):
+        return self.obj.n_fcn
+
+    @property
+    def n_grad(self):
+        return self.obj.n_grad
+
+    def to_ext(self, u):
+        x = np.array(u, dtype=np.float64)
+        x[self.b] = self.lo[self.b] + (self.hi[self.b] - self.lo[self.b]) * (np.sin(u[self.b]) + 1.0) / 2.0
+        return x
+
+    def to_int(self, x):
+        u = np.array(x, dtype=np.float64)
+        r = 2.0 * (x[self.b] - self.lo[self.b]) / (self.hi[self.b] - self.lo[self.b]) - 1.0
+        u[self.b] = np.arcsin(np.clip(r, -1.0, 1.0))
+        return u
+
+    def _chain(self, u, g):
+        g = np.array(g)
+        g[self.b] *= (self.hi[self.b] - self.lo[self.b]) * np.cos(u[self.b]) / 2.0
+        return g
+
+    def value(self, u):
+        return self.obj.value(self.to_ext(u))
+
+    def value_grad(self, u):
+        f, g = self.obj.value_grad(self.to_ext(u))
+        return f, self._chain(u, g)
+
+    def fd_grad(self, u):
+        return self._chain(u, self.obj.fd_grad(self.to_ext(u)))
+
+
+@dataclass
+class FitResult:
+    x: np.ndarray
+    fval: float
+    edm: float
+    converged: bool
+    n_iter: int
+    n_fcn: int
+    n_grad: int
+    seconds: float
+    errors: np.ndarray = field(default=None)
+    trajectory: list = field(default_factory=list)  # (x, f, g) at every accepted point
+
+
+def minimize(obj, x0, edm_tol=1e-4, max_iter=2000, scale=None, keep_trajectory=False, max_seconds=None):
+    """Variable-metric minimisation of obj from x0.
+
+    The inverse-Hessian estimate V starts diagonal (scale**2, default
+    0.01 * max(|x0|, 0.1)) and is rescaled after the first step (Shanno's
+    y.s / y.y); steps p = -V g with a backtracking line search (sufficient
+    decrease, quadratic interpolation); BFGS update when the curvature y.s is
+    positive.  Stops when EDM = g.V.g / 2 < edm_tol (MIGRAD's criterion).
+    Returns a FitResult; errors = sqrt(2 * 0.5 * diag V) for an NLL-type FCN.
+    """
+    t0 = time.perf_counter()
+    x = np.array(x0, dtype=np.float64)
+    n = len(x)
+    if scale is None:
+        scale = 0.01 * np.maximum(np.abs(x), 0.1)
+    V = np.diag(np.asarray(scale, dtype=np.float64) ** 2)
+    f, g = obj.value_grad(x)
+    traj = [(x.copy(), f, g.copy())] if keep_trajectory else []
+    first = True
+    converged = False
+    it = 0
+    edm = 0.5 * float(g @ V @ g)
+    for it in range(1, max_iter + 1):
+        p = -V @ g
+        slope = float(g @ p)
+        if slope >= 0:  # lost positive definiteness: restart from the diagonal
+            V = np.diag(np.asarray(scale, dtype=np.float64) ** 2)
+            p = -V @ g
+            slope = float(g @ p)
+        edm = -0.5 * slope
+        if edm < edm_tol:
+            converged = True
+            break
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
+        t = 1.0
+        accepted = False
+        for _ in range(30):
+            xn = x + t * p
+            if obj.mode == "analytic":
+                fn, gn = obj.value_grad(xn)
+            else:
+                fn, gn = obj.value(xn), None
+            if np.isfinite(fn) and fn <= f + 1e-4 * t * slope:
+                accepted = True
+                break
+            # quadratic interpolation of f along p, kept inside [0.1 t, 0.5 t]
+            denom = 2.0 * (fn - f - t * slope) if np.isfinite(fn) else np.inf
+            tq = -slope * t * t / denom if denom > 0 else 0.5 * t
+            t = min(max(tq, 0.1 * t), 0.5 * t)
+        if not accepted:
+            break
+        if gn is None:
+            gn = obj.fd_grad(xn)
+        s = xn - x
+        y = gn - g
+        ys = float(y @ s)
+        if ys > 1e-300:
+            if first:
+                V *= ys / float(y @ y)
+                first = False
+            rho = 1.0 / ys
+            Vy = V @ y
+            V = V - rho * (np.outer(s, Vy) + np.outer(Vy, s)) + (rho * rho * float(y @ Vy) + rho) * np.outer(s, s)
+        x, f, g = xn, fn, gn
+        if keep_trajectory:
+            traj.append((x.copy(), f, g.copy()))
+    errors = np.sqrt(np.maximum(np.diag(V), 0.0))
+    return FitResult(x, f, edm, converged, it, obj.n_fcn, obj.n_grad, time.perf_counter() - t0, errors, traj)
+
+
+def random_starts(model, n, rng, coupling_max=2.0, mass_frac=0.05, width_frac=0.5):
+    """Random start points over the floating parameters: couplings uniform in
+    modulus (0, coupling_max) and phase; masses uniform within +-mass_frac of
+    the model value; widths uniform within +-width_frac (relative).
+    Returns an (n, n_float) array."""
+    idx = np.flatnonzero(model.float_mask)
+    base = model.params[idx]
+    kinds = parameter_kinds(model)[idx]
+    out = np.empty((n, len(idx)))
+    for j in range(n):
+        x = base.copy()
+        k = 0
+        while k < len(idx):
+            if kinds[k] == "re" and k + 1 < len(idx) and kinds[k + 1] == "im":
+                r = rng.uniform(0.0, coupling_max)
+                ph = rng.uniform(-np.pi, np.pi)
+                x[k], x[k + 1] = r * np.cos(ph), r * np.sin(ph)
+                k += 2
+                continue
+            if kinds[k] == "mass":
+                x[k] = base[k] * (1.0 + rng.uniform(-mass_frac, mass_frac))
+            elif kinds[k] == "width":
+                x[k] = base[k] * (1.0 + rng.uniform(-width_frac, width_frac))
+            else:
+                x[k] = base[k] * (1.0 + rng.uniform(-0.5, 0.5))
+            k += 1
+        out[j] = x
+    return out
+
+
+def parameter_kinds(model):
+    """Kind of every entry of the fit-parameter vector: 're' / 'im' (complex
+    couplings), 'mass', 'width', 'other' (K-matrix blocks), 'fixed'."""
+    from . import abi
+    kinds = []
+    for r in model.resonances:
+        size = len(r.params)
+        k = ["other"] * size
+        ncpl = {abi.PWA_CHAIN_LAMBDA: 14, abi.PWA_CHAIN_PC: 8, abi.PWA_CHAIN_ZCS: 10}[r.chain]
+        if r.chain == abi.PWA_CHAIN_PC and r.type == abi.PWA_KMATRIX:
+            ncpl = 8
+        for q in range(0, ncpl, 2):
+            k[q], k[q + 1] = "re", "im"
+        k[size - 5] = "mass"
+        k[size - 4] = "width"
+        k[size - 3] = k[size - 2] = k[size - 1] = "fixed"
+        kinds.extend(k)
+    return np.array(kinds)
+
+
+def default_limits(model, mass_frac=0.10, width_range=(1e-3, 0.6)):
+    """Limits of the floating parameters: masses within +-mass_frac of the
+    model value, widths in width_range (GeV), everything else free."""
+    idx = np.flatnonzero(model.float_mask)
+    kinds = parameter_kinds(model)[idx]
+    base = model.params[idx]
+    lo = np.full(len(idx), -np.inf)
+    hi = np.full(len(idx), np.inf)
+    m = kinds == "mass"
+    lo[m], hi[m] = base[m] * (1 - mass_frac), base[m] * (1 + mass_frac)
+    w = kinds == "width"
+    lo[w], hi[w] = width_range
+    return lo, hi
+
+
+def fit(ctx, model, x0, mode="analytic", limits=None, edm_tol=1e-4, max_iter=2000, max_seconds=None,
+        keep_trajectory=False):
+    """One fit of the floating parameters from x0 (external units), optionally
+    within limits (lo, hi); the result's x and errors are external.  With
+    keep_trajectory, r.trajectory lists (x, fcn, gradient) of every analytic
+    evaluation of the fit in external units."""
+    obj = Objective(ctx, model.params, model.float_mask, mode, record=keep_trajectory)
+    if limits is None:
+        r = minimize(obj, x0, edm_tol=edm_tol, max_iter=max_iter, max_seconds=max_seconds)
+        r.trajectory = obj.records
+        return r
+    bo = Bounded(obj, *limits)
+    x0c = np.clip(x0, bo.lo, bo.hi)
+    r = minimize(bo, bo.to_int(x0c), edm_tol=edm_tol, max_iter=max_iter, max_seconds=max_seconds,
+                 scale=np.where(bo.b, 0.05, 0.01 * np.maximum(np.abs(x0c), 0.1)))
+    r.trajectory = obj.records
+    u = r.x
+    r.x = bo.to_ext(u)
+    dxdu = np.where(bo.b, (bo.hi - bo.lo) * np.cos(u) / 2.0, 1.0)
+    r.errors = r.errors * np.abs(dxdu)
+    return r
+
+
+def multistart(ctx, model, starts, mode="analytic", limits=None, edm_tol=1e-4, max_iter=2000, max_seconds=None,
+               keep_trajectory=False):
+    """Fits from every start (rows of `starts`, floating parameters)."""
+    return [fit(ctx, model, x0, mode, limits, edm_tol, max_iter, max_seconds, keep_trajectory) for x0 in starts]

@@ -1,0 +1,84 @@
+"""The variable-metric minimiser of pwa_amd.fit on functions with known minima
+(no GPU): an ill-conditioned quadratic, the Rosenbrock valley, and the
+finite-difference gradient mode."""
+import numpy as np
+
+from pwa_amd import models
+from pwa_amd.fit import minimize, parameter_kinds, random_starts
+
+
+class _Fn:
+    def __init__(self, f, g, mode="analytic"):
+        self.f, self.g, self.mode = f, g, mode
+        self.n_fcn = self.n_grad = 0
+
+    def value(self, x):
+        self.n_fcn += 1
+        return self.f(x)
+
+    def value_grad(self, x):
+        if self.mode == "analytic":
+            self.n_grad += 1
+            return self.f(x), self.g(x)
+        return self.value(x), self.fd_grad(x)
+
+    def fd_grad(self, x):
+        out = np.empty_like(x)
+        for k in range(len(x)):
+            h = 1e-6 * max(abs(x[k]), 1e-2)
+            e = np.zeros_like(x)
+            e[k] = h
+            out[k] = (self.value(x + e) - self.value(x - e)) / (2 * h)
+        return out
+
+
+def test_quadratic_exact_minimum_and_errors():
+    rng = np.random.default_rng(1)
+    n = 12
+    Q = rng.normal(size=(n, n))
+    A = Q @ Q.T + np.diag(np.logspace(-2, 2, n))
+    xs = rng.normal(size=n)
+    f = lambda x: 0.5 * (x - xs) @ A @ (x - xs)
+    g = lambda x: A @ (x - xs)
+    r = minimize(_Fn(f, g), np.zeros(n), edm_tol=1e-12, scale=np.ones(n))
+    assert r.converged
+    assert np.max(np.abs(r.x - xs)) < 1e-5
+    # the variable-metric estimate approaches A^-1 (the parabolic errors)
+    assert np.allclose(r.errors, np.sqrt(np.diag(np.linalg.inv(A))), rtol=0.2)
+
+
+def test_rosenbrock():
+    f = lambda x: (1 - x[0]) ** 2 + 100 * (x[1] - x[0] ** 2) ** 2
+    g = lambda x: np.array([-2 * (1 - x[0]) - 400 * x[0] * (x[1] - x[0] ** 2), 200 * (x[1] - x[0] ** 2)])
+    r = minimize(_Fn(f, g), np.array([-1.2, 1.0]), edm_tol=1e-14, scale=np.ones(2))
+    assert r.converged
+    assert np.allclose(r.x, [1.0, 1.0], atol=1e-5)
+
+
+def test_fd_mode_matches_analytic_mode():
+    rng = np.random.default_rng(2)
+    n = 6
+    A = np.diag(rng.uniform(0.5, 5.0, n))
+    xs = rng.normal(size=n)
+    f = lambda x: 0.5 * (x - xs) @ A @ (x - xs) + 0.1 * np.sum((x - xs) ** 4)
+    g = lambda x: A @ (x - xs) + 0.4 * (x - xs) ** 3
+    ra = minimize(_Fn(f, g, "analytic"), np.zeros(n), edm_tol=1e-12, scale=np.ones(n))
+    rf = minimize(_Fn(f, g, "fd"), np.zeros(n), edm_tol=1e-12, scale=np.ones(n))
+    assert ra.converged and rf.converged
+    assert np.max(np.abs(ra.x - rf.x)) < 1e-5
+    assert rf.n_fcn > ra.n_grad  # the plain mode pays 2P evaluations per gradient
+
+
+def test_random_starts_cover_floating_parameters():
+    m = models.config_small(1)
+    kinds = parameter_kinds(m)
+    assert len(kinds) == len(m.params)
+    fl = np.flatnonzero(m.float_mask)
+    st = random_starts(m, 50, np.random.default_rng(3))
+    assert st.shape == (50, len(fl))
+    k = kinds[fl]
+    re = np.flatnonzero(k == "re")
+    mod = np.hypot(st[:, re], st[:, re + 1])
+    assert mod.max() <= 2.0 and mod.min() >= 0.0
+    mass = np.flatnonzero(k == "mass")
+    assert np.all(np.abs(st[:, mass] / m.params[fl][mass] - 1) <= 0.05 + 1e-12)
