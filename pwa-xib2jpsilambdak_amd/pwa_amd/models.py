@@ -238,6 +238,17 @@ def config_small(seed=1) -> Model:
     return m
 
 
+def config_tiny(seed=4) -> Model:
+    """2 Lambda* relativistic Breit-Wigners (J^P 1/2-, 3/2+), m ~ U(1.40, 2.35) GeV,
+    Gamma ~ U(0.02, 0.30) GeV: a small model for fit-procedure tests."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    res = [lambda_res(rng, s2, p, rng.uniform(1.40, 2.35), rng.uniform(0.02, 0.30)) for s2, p in [(1, -1), (3, 1)]]
+    m = assemble(d, res, "tiny: 2 Lambda* BW")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def _nominal_lambda(rng):
     return [lambda_res(rng, s2, p, rng.uniform(1.40, 2.58), rng.uniform(0.02, 0.30))
             for s2, p in NOMINAL_LAMBDA_JP]

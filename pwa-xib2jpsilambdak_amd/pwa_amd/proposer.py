@@ -25,33 +25,38 @@ class Proposer:
     def _norm_x(self, X):
         return (X - self.mu_x) / self.sd_x
 
-    def train(self, X, F, G, epochs=200, batch=256, lr=1e-3, seed=0):
+    def train(self, X, F, G, epochs=300, batch=512, lr=1e-3, seed=0):
+        """Fits the network to the FCN and its gradient.  The FCN target is
+        compressed, u = (F - F_lo) / s, y = asinh(u) (F_lo = the 2nd
+        percentile, s = the 10-50 percentile spread), so the low region of
+        the surface carries most of the weight; the gradient target is the
+        derivative of y with respect to the standardised inputs."""
         X = np.asarray(X, dtype=np.float64)
         F = np.asarray(F, dtype=np.float64)
         G = np.asarray(G, dtype=np.float64)
         self.mu_x = X.mean(axis=0)
         self.sd_x = np.maximum(X.std(axis=0), 1e-6)
-        # robust FCN scale: the samples span many orders of the FCN; train on
-        # a clipped, standardised value so the low region dominates
-        self.f_ref = np.percentile(F, 50)
-        self.sd_f = max(np.percentile(F, 90) - np.percentile(F, 10), 1e-6)
-        z = self._norm_x(X)
-        yf = np.clip((F - self.f_ref) / self.sd_f, -5.0, 5.0)
-        yg = np.clip(G * self.sd_x / self.sd_f, -5.0, 5.0)
-        y = np.concatenate([yf[:, None], yg], axis=1)
+        self.f_lo = np.percentile(F, 2)
+        self.s_f = max(np.percentile(F, 50) - np.percentile(F, 10), 1e-6)
+        u = (F - self.f_lo) / self.s_f
+        yf = np.arcsinh(u)
+        yg = G * (self.sd_x / self.s_f) / np.sqrt(1.0 + u * u)[:, None]
+        y = np.concatenate([yf[:, None], np.clip(yg, -10.0, 10.0)], axis=1)
         ow = np.concatenate([[1.0], np.full(self.P, self.grad_weight / self.P)])
-        self.net.set_data(z, y, ow)
+        self.net.set_data(self._norm_x(X), y, ow)
         return self.net.train(epochs, batch=min(batch, len(X)), lr=lr, seed=seed)
 
     def predict(self, X):
+        """Predicted FCN at X (the inverse of the target compression)."""
         out = self.net.predict(self._norm_x(np.asarray(X, dtype=np.float64))).astype(np.float64)
-        return self.f_ref + self.sd_f * out[:, 0], out[:, 1:] * self.sd_f / self.sd_x
+        return self.f_lo + self.s_f * np.sinh(out[:, 0])
 
-    def propose(self, starts, n_top, steps=100, eta=0.05):
+    def propose(self, starts, n_top, steps=200, eta=0.02, min_dist=0.5):
         """Descend every start on the learned surface (normalised
         coordinates, steps of eta along the normalised predicted gradient,
-        kept within the limits) and return the n_top end points with the
-        lowest predicted FCN, best first."""
+        kept within the limits) and return up to n_top end points with the
+        lowest predicted FCN, best first, at least min_dist apart (normalised
+        Euclidean distance / sqrt(P))."""
         z = self._norm_x(np.asarray(starts, dtype=np.float64))
         zlo, zhi = self._norm_x(self.lo), self._norm_x(self.hi)
         for _ in range(steps):
@@ -60,5 +65,11 @@ class Proposer:
             g /= np.maximum(np.sqrt(np.mean(g * g, axis=1, keepdims=True)), 1e-6)
             z = np.clip(z - eta * g, zlo, zhi)
         score = self.net.predict(z)[:, 0]
-        order = np.argsort(score)[:n_top]
-        return self.mu_x + self.sd_x * z[order], score[order]
+        picked = []
+        for j in np.argsort(score):
+            if all(np.linalg.norm(z[j] - z[k]) / np.sqrt(self.P) > min_dist for k in picked):
+                picked.append(j)
+            if len(picked) == n_top:
+                break
+        picked = np.array(picked, dtype=int)
+        return self.mu_x + self.sd_x * z[picked], self.f_lo + self.s_f * np.sinh(score[picked])

@@ -8,8 +8,11 @@ Procedure (all fits within the same parameter limits):
   1. n_random random starts fitted with the analytic gradient: the best FCN
      of all fits made here defines the global minimum F*; a fit reaches it
      when its FCN < F* + 0.1.  p_random = fraction of random starts that do.
-     (Analytic and plain fits from one start reach the same minimum; the
-     analytic ones are used to count cheaply.)
+     Both gradient modes run the same minimiser, so they sample the same
+     basins of attraction; single trajectories from one start may still end
+     in different minima (the finite-difference gradient's truncation error
+     changes the path), so p_random is a property of the start distribution
+     and the landscape, estimated with the cheap analytic fits.
   2. n_plain random starts fitted in the plain mode: mean time t_plain.
      Expected wall-time of the plain procedure = t_plain / p_random.
   3. Accelerated: n_train analytic fits from random starts record every
@@ -46,9 +49,10 @@ def setup(model, nd, nm, seed):
     return ctx
 
 
-def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=48, n_top=32, n_rank=4096,
-        epochs=150, seed=11, fit_seconds=20.0, plain_seconds=120.0):
-    model = {1: models.config_small, 2: models.config_nominal, 3: models.config_kmatrix}[config]()
+def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=64, n_top=32, n_rank=4096,
+        epochs=300, seed=11, fit_seconds=20.0, plain_seconds=120.0):
+    model = {0: models.config_tiny, 1: models.config_small, 2: models.config_nominal,
+             3: models.config_kmatrix}[config]()
     ctx = setup(model, ndata, nmc, seed)
     lim = default_limits(model)
     rng = np.random.default_rng(seed + 5)
@@ -72,7 +76,7 @@ def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=48
         plain.append(fit(ctx, model, x0, "fd", lim, max_iter=1000, max_seconds=plain_seconds))
         _log(f"plain fit {j + 1}/{n_plain}: fcn {plain[-1].fval:.4f} ({plain[-1].seconds:.1f} s)")
     t_plain = np.array([r.seconds for r in plain])
-    agree = [abs(p.fval - a.fval) < 1e-3 for p, a in zip(plain, rand) if p.converged and a.converged]
+    agree = [abs(p.fval - a.fval) < 1e-3 for p, a in zip(plain, rand)]
 
     # 3. accelerated procedure
     t_acc0 = time.perf_counter()
@@ -106,6 +110,12 @@ def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=48
     ok_rand = f_rand < f_star + 0.1
     ok_ref = np.array([r.fval < f_star + 0.1 for r in refits])
     p_rand = float(ok_rand.mean())
+    f_ref = np.array([r.fval for r in refits])
+    # a looser yardstick: the best 5 % of the random-start minima
+    f_q05 = float(np.percentile(f_rand, 5))
+    good = {"threshold_fcn": f_q05, "random": float(np.mean(f_rand <= f_q05)),
+            "proposer": float(np.mean(f_ref <= f_q05)),
+            "median_fcn_random": float(np.median(f_rand)), "median_fcn_proposer": float(np.median(f_ref))}
     # accelerated wall-time: data + training + ranking + refits up to the first success
     if ok_ref.any():
         k = int(np.argmax(ok_ref))
@@ -125,7 +135,9 @@ def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=48
         "proposer": {"train_fits": n_train, "samples": len(X), "train_s": t_train, "data_s": t_data,
                      "rank_s": t_rank, "top": n_top, "refit_success_fraction": float(ok_ref.mean()),
                      "first_success_rank": int(np.argmax(ok_ref)) if ok_ref.any() else None},
+        "best_5pct_of_minima": good,
         "gain": {"plain_over_accelerated": t_plain_proc / t_acc if np.isfinite(t_acc) else None,
+                 "plain_over_analytic_random": t_plain_proc / t_an_proc if np.isfinite(t_an_proc) else None,
                  "proposer_success_vs_random": float(ok_ref.mean()) / p_rand if p_rand > 0 else None},
         "seconds_random_fits": t_rand,
     })
@@ -140,9 +152,9 @@ def main():
     ap.add_argument("--nmc", type=int, default=100000)
     ap.add_argument("--random", type=int, default=1000)
     ap.add_argument("--plain", type=int, default=12)
-    ap.add_argument("--train-fits", type=int, default=48)
+    ap.add_argument("--train-fits", type=int, default=64)
     ap.add_argument("--top", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--seed", type=int, default=11)
     a = ap.parse_args()
     r = run(a.config, a.ndata, a.nmc, a.random, a.plain, a.train_fits, a.top, epochs=a.epochs, seed=a.seed)
