@@ -51,55 +51,77 @@ def coupling_counts(d):
     return k_lx, k_pc
 
 
-STAGES = ("lam", "pcang", "shape", "adj_lam", "adj_pc")
+STAGES = ("lam", "pcang", "shape", "direct", "adj_lam", "adj_pc")
+H_ROWS = 8  # helicity amplitudes per event
 
 
 def work_model(model):
-    """Per-event algorithmic work of each stage kernel (FP64 flops, complex
-    MAC = 8, complex mul = 6; transcendentals not counted).  The coherent
-    contraction F_fwd = 8 H K = 64 K and its adjoint F_adj = 2 F_fwd are split
-    by coupling class between the kernels that do them:
-      k_lam     64 K_lx                      (Lambda* + Zcs contraction)
-      k_pcang   64 K_pc + 8 H R^2            (Pc contraction, quadratic-form Gram terms)
-      k_shape   8 S R^2 + 299 S n_cls        (smeared events: sub-bin quadratic
-                                              forms and K-matrix row-0 solves)
-      k_adj_lam 128 K_lx,  k_adj_pc 128 K_pc
-    K = complex helicity couplings, H = 8 helicity rows, R = Pc-chain
-    resonances, S = sub-bins, n_cls = K-matrix J^P classes."""
+    """Algorithmic work per event (FP64 flops; complex MAC = 8, complex mul =
+    6; transcendentals and lineshape arithmetic not counted), counted for the
+    reference's algorithm:
+      F_fwd   = 32 K            coherent contraction: each of the K couplings of
+                                the reference's coupling vector enters 4 of the 8
+                                helicity amplitudes with one complex MAC
+      F_adj   = 2 F_fwd         coupling and lineshape-parameter adjoints
+      F_smear = 8 S R^2 + 8 H R^2   smeared events: sub-bin quadratic forms and
+                                their Gram terms (S sub-bins, R Pc resonances, H = 8)
+      F_KM    = 299 S n_cls     K-matrix row-0 solves (S = 1 when unsmeared)
+    Each term is attributed to the stage kernel that does it; `flops` returns
+    a stage's work over one sample launch."""
     d = model.desc
     from pwa_amd._lib import helicity_couplings
-    K = len(helicity_couplings(d, model.params))  # complex couplings
+    K = len(helicity_couplings(d, model.params))
     k_lx, k_pc = coupling_counts(d)
     assert k_lx + k_pc == K, (k_lx, k_pc, K)
     R = d.n_pc
     ncls = len({(d.spin2[r], d.parity[r]) for r in range(d.n_lambda, d.n_lambda + d.n_pc)
                 if d.type[r] == abi.PWA_KMATRIX})
+    nb = d.scheme_nbins[0] if d.n_schemes else 1
 
-    def flops(stage, n_events, nbins, n_smeared):
-        """Work of one launch over n_events events of which n_smeared are smeared."""
-        per = {"lam": 64.0 * k_lx, "pcang": 64.0 * k_pc + 64.0 * R * R, "shape": 0.0,
-               "adj_lam": 128.0 * k_lx, "adj_pc": 128.0 * k_pc}[stage]
-        f = per * n_events
+    def flops(stage, launch, n_smeared):
+        n = launch["n"]
+        direct = launch["ms_direct"] > 0.0          # unsmeared events took k_pcdirect
+        ns = n_smeared if direct else n              # events of the quadratic-form path
+        nu = n - n_smeared                           # unsmeared events
+        km_s = 299.0 * ncls * (nb * n_smeared + nu)
+        if stage == "lam":
+            return 32.0 * k_lx * n
+        if stage == "pcang":
+            return (32.0 * k_pc + 8.0 * H_ROWS * R * R) * ns if R else 0.0
         if stage == "shape":
-            f += (8.0 * nbins * R * R + 299.0 * nbins * ncls) * n_smeared
-        return f
+            return 8.0 * nb * R * R * n_smeared + (km_s if not direct else 299.0 * ncls * nb * n_smeared)
+        if stage == "direct":
+            return ((32.0 + (64.0 if launch["grad"] else 0.0)) * k_pc + 299.0 * ncls) * nu if direct else 0.0
+        if stage == "adj_lam":
+            return 64.0 * k_lx * n
+        if stage == "adj_pc":
+            return 64.0 * k_pc * ns
+        raise KeyError(stage)
 
     return K, flops
 
 
-def traffic_per_launch(stage, n_events, launches_per_step):
-    """HBM bytes per launch of a stage kernel from the per-event figure that
-    the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded (profiles/hbm_traffic.json,
-    already scaled by the copy-kernel calibration); None without it."""
+def traffic_per_eval(launches, n_smeared):
+    """HBM bytes of one evaluation's stage kernels from the per-event figures
+    that the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded
+    (profiles/hbm_traffic.json, scaled by the copy-kernel calibration, per
+    event each kernel processes); None without them."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        t = json.load(f)
-    per_event = t.get("bytes_per_event", {}).get("k_" + stage)
-    if per_event is None:
-        return None
-    return round(per_event * n_events / max(launches_per_step, 1))
+        per = json.load(f).get("bytes_per_event", {})
+    total = 0.0
+    for L in launches:
+        n, ns = L["n"], n_smeared[L["kind"]]
+        direct = L["ms_direct"] > 0.0
+        counts = {"k_lam": n, "k_pcang": ns if direct else n, "k_shape": ns if direct else n,
+                  "k_pcdirect": n - ns if direct else 0, "k_adj_lam": n, "k_adj_pc": ns if direct else n}
+        for k, cnt in counts.items():
+            if cnt and k not in per:
+                return None
+            total += per.get(k, 0.0) * cnt
+    return round(total)
 
 
 def build_workload(nd, nm, seed=2024):
@@ -200,32 +222,34 @@ def main():
     print(f"[bench] forward: {t_fwd * 1e3:.3f} ms/eval -> FD gradient {1 / fd_rate:.3f} s", file=sys.stderr,
           flush=True)
 
-    # --- roofline of the dominant stage kernel: its device time summed over
-    #     all its launches of the timed evaluations (HIP events on the
-    #     context's stream, recorded around each kernel)
+    # --- roofline: the stage kernels of one evaluation (HIP events on the
+    #     context's stream around each kernel), algorithmic work as work_model
     K, flops = work_model(model)
-    nb = model.desc.scheme_nbins[0]
     n_smeared = {abi.PWA_DATA: int(np.sum(ds > 0)), abi.PWA_MC: int(np.sum(ms > 0))}
-    tot = {st: [0.0, 0.0, 0] for st in STAGES}  # ms, flops, launches
+    tot = {st: [0.0, 0.0] for st in STAGES}  # ms, flops over the timed steps
     for step in launches:
         for L in step:
-            for st in tot:
-                t_ms = L["ms_" + st]
-                if t_ms <= 0.0:
-                    continue
-                tot[st][0] += t_ms
-                tot[st][1] += flops(st, L["n"], nb, n_smeared[L["kind"]])
-                tot[st][2] += 1
-    dom = max(tot, key=lambda st: tot[st][0])
-    ms_sum, fl_sum, nlaunch = tot[dom]
-    avg_ms = ms_sum / nlaunch
-    achieved = fl_sum / (ms_sum * 1e-3) / 1e12
-    share = {st: round(v[0] / args.steps, 4) for st, v in tot.items()}
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic_per_launch(dom, nd + nm, nlaunch // args.steps),
-            "kernel": f"k_{dom} (gradient pipeline; data and MC sample launches)",
-            "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": fl_sum / nlaunch,
-            "stage_ms_per_step": share, "K_couplings": K}
+            for st in STAGES:
+                tot[st][0] += L["ms_" + st]
+                tot[st][1] += flops(st, L, n_smeared[L["kind"]])
+    t_pipe = sum(v[0] for v in tot.values()) / args.steps          # ms per evaluation
+    f_pipe = sum(v[1] for v in tot.values()) / args.steps          # flops per evaluation
+    achieved = f_pipe / (t_pipe * 1e-3) / 1e12
+    stages = {}
+    for st, (t_ms, fl) in tot.items():
+        if t_ms <= 0.0:
+            continue
+        tf = fl / (t_ms * 1e-3) / 1e12
+        stages["k_" + st] = {"ms": round(t_ms / args.steps, 4), "tflops": round(tf, 3),
+                             "frac": round(tf / PEAK_FP64, 4)}
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    roof = {"bound": "mfma", "compute": "FP64 (vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
+            "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64, 4), "traffic": traffic_per_eval(launches[-1], n_smeared),
+            "kernel": "gradient pipeline of one evaluation (k_lam, k_pcang, k_shape, k_pcdirect, k_adj_lam, "
+                      "k_adj_pc over the data and MC samples)",
+            "avg_launch_ms": round(t_pipe, 4), "flops_per_launch": f_pipe, "dominant_kernel": dom,
+            "stages": stages, "K_couplings": K}
 
     frac_smeared = float(np.mean(ms > 0))
     out = {

@@ -256,14 +256,16 @@ int pwa_last_timing(pwa_ctx* ctx, double* ms_total, double* ms_main);
 
 /* Per-sample device times of the last evaluation (HIP events on the
    context's stream).  Launch q is the stage pipeline over one sample (data,
-   then MC); ms[5q .. 5q+4] are the times of its five kernels in launch
+   then MC); ms[6q .. 6q+5] are the times of its six kernels in launch
    order: Lambda* and Zcs amplitude (k_lam, with k_lsgen when it runs), Pc
-   angular sums (k_pcang), Pc lineshapes and likelihood term (k_shape),
-   Lambda* and Zcs adjoint (k_adj_lam), Pc adjoint (k_adj_pc); kernels that
-   did not run read 0.  kind is the sample, scheme its smearing scheme when
-   it has one (else -1), is_grad whether the gradient kernels ran, nevents
-   its event count.  *n receives the number of launches (up to cap are
-   written; ms holds 5 cap values). */
+   angular sums of the smeared events (k_pcang), Pc sub-bin lineshapes and
+   likelihood term of the smeared events (k_shape), direct Pc amplitude,
+   likelihood term and Pc adjoint of the unsmeared events (k_pcdirect),
+   Lambda* and Zcs adjoint (k_adj_lam), Pc adjoint of the smeared events
+   (k_adj_pc); kernels that did not run read 0.  kind is the sample, scheme
+   its smearing scheme when it has one (else -1), is_grad whether the
+   gradient kernels ran, nevents its event count.  *n receives the number of
+   launches (up to cap are written; ms holds 6 cap values). */
 int pwa_last_launches(pwa_ctx* ctx, int32_t cap, int32_t* n, double* ms, int32_t* kind,
                       int32_t* scheme, int32_t* is_grad, int64_t* nevents);
 

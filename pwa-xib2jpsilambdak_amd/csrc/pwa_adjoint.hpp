@@ -349,4 +349,44 @@ __device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict_
         }
 }
 
+// Pc resonance r of an unsmeared event, whose seed is rank one,
+// U_r = L_r conj(A): chain_adjoint(U_r) = L_r T0 with T0 = chain_adjoint(conj(A))
+// shared by all resonances, so the coupling adjoint is that of
+// pc_coupling_adjoint with T = L_r T0.  Returns P_r = sum_h conj(A_h) S_rh
+// = sum_v T0_v V_rv (V_r: pc_vtable), the factor of the lineshape-parameter
+// derivatives d|A|^2 = 2 Re(dL_r P_r).
+__device__ __forceinline__ C64 pc_direct_adjoint(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
+                                                 const C64 T0[2][3][2], double cZ, double sZ, double cP, double sP,
+                                                 C64 L, double a2, const GradLayout& G, Emitter& em) {
+    const int J2 = S->spin2[r];
+    double dz[2][4];
+    pc_dz(J2, cZ, sZ, dz);
+    const C64* c = cpl + S->ampcount[r];
+    const C64 La = L * a2;
+    C64 P;
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int ps = 0; ps < 3; ++ps) {
+            em.begin(8);
+            const int n0 = n;
+#pragma unroll
+            for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+                for (int ip = 0; ip < 2; ++ip) {
+                    const int n2 = 2 * (ps - 1) - (2 * ip - 1);
+                    if (n2 > J2 || n2 < -J2) continue;
+                    const C64 t = T0[k][ps][iz] * (dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2]);
+                    cmac(P, t, c[n]);
+                    const C64 z = La * t;
+                    em.put(2 * (n - n0), z.re);
+                    em.put(2 * (n - n0) + 1, -z.im);
+                    ++n;
+                }
+            em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
+        }
+    return P;
+}
+
 } // namespace pwa

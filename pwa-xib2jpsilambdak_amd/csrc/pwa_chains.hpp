@@ -218,15 +218,19 @@ __device__ __forceinline__ ChainAng load_chain(const double* ev, int64_t ld, int
     return a;
 }
 
-// Pc angular-times-coupling sums (no lineshape) for resonance r
-// (src/kernel_calEva.cu:610-662; tables of filldlz src/RooDalitzAmplitude.cxx:7189-7245).
-__device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
-                                           const ChainAng& a, double cZ, double sZ, double cP,
-                                           double sP, C64 out[8]) {
+// Pc coupling table of resonance r: V[k][ps][iz] += s * sum over the
+// resonance's couplings c_n of dfunhalf(theta_p) dfun(theta_Z) c_n
+// (src/kernel_calEva.cu:610-662; tables of filldlz
+// src/RooDalitzAmplitude.cxx:7189-7245), with s = 1 unless SCALED.
+// chain_contract(V) is then the resonance's angular-times-coupling sum; the
+// map is linear, so a lineshape-weighted sum over resonances is contracted
+// once.
+template <bool SCALED>
+__device__ __forceinline__ void pc_vtable(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
+                                          double cZ, double sZ, double cP, double sP, C64 s, C64 V[2][3][2]) {
     const int J2 = S->spin2[r];
     double dz[2][4];
     pc_dz(J2, cZ, sZ, dz);
-    C64 V[2][3][2];
     const C64* c = cpl + S->ampcount[r];
     int n = 0;
 #pragma unroll
@@ -240,9 +244,18 @@ __device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, cons
                     const int n2 = 2 * (ps - 1) - (2 * ip - 1);  // 2(helpsi - help)
                     if (n2 > J2 || n2 < -J2) continue;
                     const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
-                    rmac(V[k][ps][iz], w, c[n]);
+                    if constexpr (SCALED) cmac(V[k][ps][iz], s * w, c[n]);
+                    else rmac(V[k][ps][iz], w, c[n]);
                     ++n;
                 }
+}
+
+// Pc angular-times-coupling sums (no lineshape) for resonance r.
+__device__ __forceinline__ void pc_angular(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
+                                           const ChainAng& a, double cZ, double sZ, double cP,
+                                           double sP, C64 out[8]) {
+    C64 V[2][3][2];
+    pc_vtable<false>(S, cpl, r, cZ, sZ, cP, sP, C64(1.0, 0.0), V);
     chain_contract(a, V, out);
 }
 

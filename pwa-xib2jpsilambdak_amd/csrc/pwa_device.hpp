@@ -97,16 +97,18 @@ struct SegDesc {
     int32_t pad;
 };
 
-// Block -> segment view of a reducing-kernel launch over a whole sample.
+// Block -> segment view of a per-segment kernel launch over a whole sample.
 struct SegView {
     const SegDesc* segs;
     const int32_t* blkmap;   // segment of each block
     int32_t force_unsmeared; // evaluate every segment with scheme 0
+    int32_t direct_ok;       // unsmeared segments take the direct Pc path (k_pcdirect)
     __device__ __forceinline__ SegDesc seg() const {
         SegDesc s = segs[blkmap[blockIdx.x]];
         if (force_unsmeared) s.scheme = 0;
         return s;
     }
+    __device__ __forceinline__ bool direct(const SegDesc& s) const { return direct_ok && s.scheme == 0; }
 };
 
 // Arguments of the stage pipeline over one sample (pwa_stages.hpp).
@@ -115,6 +117,7 @@ struct StageArgs {
     int64_t ld, n;        // sample planes and event count
     SegView sv;
     int nblocks;          // blocks of the reducing kernels (sum of the segments' nblk)
+    int passes;           // max over segments of ceil(n / (nblk kBlock)): strided passes of a block
     bool mc, grad;
     bool full;            // K-matrix members or generic Pc lineshapes present (full k_shape variant)
     int noint;
@@ -125,7 +128,8 @@ struct StageArgs {
     double* fx;           // per-event output (device order) or null
     Partial* part;        // nblocks partial sums
     int ncpl, nres, ng;   // gradient layout
-    double* gpart;        // gradient partial rows: block b of k_shape / k_adj_lam / k_adj_pc -> rows rpb b + 0 / 1 / 2
+    double* gpart;        // gradient partial rows: block b of k_shape or k_pcdirect / k_adj_lam / k_adj_pc
+                          // -> rows rpb b + 0 / 1 / 2
     int rpb;              // gradient rows per block (2, or 3 with Pc-chain resonances)
     hipEvent_t* tev;      // null, or kStageEvents events recorded around the stage kernels
     double* gen;          // generic-lineshape buffer (6 ngen planes x ld), null when ngen = 0
@@ -133,7 +137,7 @@ struct StageArgs {
 };
 
 // Stage kernels of one sample, in launch order (timing slots).
-enum Stage { ST_LAM = 0, ST_PCANG, ST_SHAPE, ST_ADJ_LAM, ST_ADJ_PC, ST_COUNT };
+enum Stage { ST_LAM = 0, ST_PCANG, ST_SHAPE, ST_DIRECT, ST_ADJ_LAM, ST_ADJ_PC, ST_COUNT };
 constexpr int kStageEvents = ST_COUNT + 1;
 
 hipError_t launch_lam(const StageArgs& a, double* pcs, double* amp, bool store_h0, hipStream_t st);

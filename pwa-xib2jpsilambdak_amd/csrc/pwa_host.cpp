@@ -33,6 +33,7 @@ struct Sample {
     SegDesc* dsegs = nullptr;           // device copy
     int32_t* dblkmap = nullptr;         // reduction block -> segment
     int nblocks = 0;                    // reduction blocks over the sample
+    int passes = 1;                     // max over segments of ceil(n / (nblk kBlock))
     int n_toys = 1;                     // data sets in the sample (toy batches)
     std::vector<int32_t> toy_blk;       // first reduction block of each toy, then nblocks
     std::vector<double> toy_sumw;       // per toy: sum of weights
@@ -300,7 +301,7 @@ void collect_times(pwa_ctx* c) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->e0, c->e1) == hipSuccess) c->last_ms = ms;
     for (int q = 0; q < c->nlaunch; ++q) {
-        const int last = c->l_grad[q] ? ST_COUNT : ST_SHAPE + 1;  // stage kernels that ran
+        const int last = c->l_grad[q] ? ST_COUNT : ST_DIRECT + 1;  // stage kernels that ran
         float t = 0.f;
         c->l_ms[q] = (hipEventElapsedTime(&t, c->le[q][0], c->le[q][last]) == hipSuccess) ? t : -1.f;
         for (int k = 0; k < ST_COUNT; ++k) {
@@ -413,11 +414,13 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.ev = s.ev;
         a.ld = s.ld;
         a.n = s.n;
-        a.sv = SegView{s.dsegs, s.dblkmap, force_unsmeared ? 1 : 0};
+        a.sv = SegView{s.dsegs, s.dblkmap, force_unsmeared ? 1 : 0, 0};
         a.nblocks = s.nblocks;
+        a.passes = s.passes;
         a.mc = (k == PWA_MC) && !mc_as_data;
         a.grad = grad;
         a.full = c->hs.km_nclass > 0 || c->pc_generic;
+        a.sv.direct_ok = (c->desc.n_pc > 0 && !a.full) ? 1 : 0;
         a.noint = noint;
         a.S = c->dS;
         a.D = c->dD;
@@ -519,6 +522,9 @@ int upload_sample(pwa_ctx* c, int kind, int64_t n, const double* cols, int64_t l
         blkmap.insert(blkmap.end(), g.nblk, (int32_t)q);
     }
     s.nblocks = (int)blkmap.size();
+    s.passes = 1;
+    for (const SegDesc& g : s.segs)
+        s.passes = std::max<int>(s.passes, (int)((g.n + (int64_t)g.nblk * kBlock - 1) / ((int64_t)g.nblk * kBlock)));
     s.toy_blk.assign(n_toys + 1, 0);
     {
         size_t q = 0;

@@ -122,7 +122,7 @@ PWA_HD C64 ls_value(const LsConst& c, const ChainKin& k, const LsEv& e) {
     if (c.kind == LS_VERTEX) return C64(c.C[0] * e.GB, 0.0);
     const double a = c.m2[0] - k.s;
     const double b = c.K[0] * e.E;
-    const double t = c.C[0] * e.G * rcp_nr(a * a + b * b);
+    const double t = (c.C[0] * e.G) * rcp_nr(a * a + b * b);
     return C64(a * t, b * t);
 }
 
@@ -140,7 +140,8 @@ PWA_HD void ls_grad(const LsConst& c, const ChainKin& k, const LsEv& e, C64& v, 
     const C64 w(a * inv, b * inv);  // 1 / (a - i b)
     const C64 w2 = w * w;
     const double cg = c.C[0] * e.G;
-    v = w * cg;
+    const double t = cg * inv;  // the value exactly as ls_value forms it
+    v = C64(a * t, b * t);
     // d w = w^2 (d a - i d b) * (-1)
     const C64 dw0 = w2 * C64(-c.m2[1], c.K[1] * e.E);
     const C64 dw1 = w2 * C64(-c.m2[2], c.K[2] * e.E);

@@ -1,7 +1,12 @@
 // Per-event stage kernels of the likelihood that depend on the number of
-// Pc-chain resonances NZ (k_pcang, k_shape, k_adj_pc) and the per-segment
-// launcher that runs them with the NZ-independent Lambda* / Zcs kernels
-// (pwa_lambda.hip):  k_lam -> k_pcang -> k_shape [-> k_adj_lam -> k_adj_pc].
+// Pc-chain resonances NZ (k_pcang, k_shape, k_pcdirect, k_adj_pc) and the
+// per-sample launcher that runs them with the NZ-independent Lambda* / Zcs
+// kernels (pwa_lambda.hip):
+//   k_lam -> k_pcang -> k_shape -> k_pcdirect [-> k_adj_lam -> k_adj_pc]
+// Smeared segments go through k_pcang / k_shape / k_adj_pc (quadratic form
+// over the sub-bins); unsmeared segments, when every Pc lineshape has the
+// factorised form, through k_pcdirect alone (direct amplitude, rank-one
+// adjoint).  Each kernel returns at once on the blocks of the other path.
 // Each NZ is instantiated in its own translation unit (pwa_stages_nz.hip
 // compiled with -DPWA_NZ=k) so the kernels build in parallel.
 #pragma once
@@ -11,6 +16,15 @@
 #include "pwa_chains.hpp"
 
 namespace pwa {
+
+// A pointer the compiler must treat as unknown on every loop iteration: the
+// coupling loads of the per-segment loops stay inside the loop instead of
+// being hoisted and held in registers across it.
+template <class T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
 
 // ---------------------------------------------------- intermediate planes
 // pcs (quadratic form):  [H02, HNI, p0 (2 NZ), Md (NZ), Mo (NZ(NZ-1))]  -> 2 + 2 NZ + NZ^2
@@ -23,11 +37,16 @@ namespace pwa {
 // coefficients p0_r = sum_h conj(H0_h) S_rh, M_qr = sum_h conj(S_qh) S_rh
 // (src/kernel_calEva.cu:610-662, 786-810); S_r kept for the gradient.
 template <int NZ, bool GRAD>
-__global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev, int64_t ld, int64_t n,
+__global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                  const DevStatic* __restrict__ S, const C64* __restrict__ cpl,
                                                  double* __restrict__ pcs, double* __restrict__ amp) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    // block (x, y) takes the events of pass y of the segment's block-strided
+    // walk (a grid of nblocks x max passes), so no loop keeps state alive
+    const SegDesc sd = sv.seg();
+    if (sv.direct(sd)) return;
+    const int64_t gi = ((int64_t)(blockIdx.x - sd.blk0) + (int64_t)blockIdx.y * sd.nblk) * kBlock + threadIdx.x;
+    if (gi >= sd.n) return;
+    const int64_t i = sd.begin + gi;
     auto P = [&](int p) -> double& { return pcs[(int64_t)p * ld + i]; };
     const ChainAng a = load_chain(ev, ld, i, E_Z_CB);
     const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
@@ -66,6 +85,36 @@ __global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev,
     }
 }
 
+// Likelihood terms of one event from its intensity f and no-interference
+// intensity fni (src/kernel_calEva.cu:964-967, 1352-1361): data v0 = w ln f
+// (f <= 0: the penalty when flagged, else 0), v1 = [f <= 0], adjoint weight
+// a = w / f; MC v0 = w f (f for an unweighted event, w <= -1000), v1 = w fni
+// when the no-interference penalty is on, a = w.  Lanes past the segment end
+// give zeros; fx_out (when set) receives f (data) or v0 (MC).
+template <bool MC>
+__device__ __forceinline__ void event_terms(const DevStatic* __restrict__ S, double f, double fni, double w,
+                                            int noint, bool valid, double* __restrict__ fx_out, int64_t i,
+                                            double& v0, double& v1, double& aw) {
+    if (MC) {
+        const bool unw = (w <= -1000.0);
+        v0 = unw ? f : f * w;
+        v1 = (noint && !unw) ? fni * w : 0.0;
+        aw = unw ? 1.0 : w;
+        if (valid && fx_out) fx_out[i] = v0;
+    } else {
+        if (f > 0) v0 = log(f) * w;
+        else v0 = (S->flags & PWA_F_NEG_FX_PENALTY) ? -1e12 : 0.0;
+        v1 = (f > 0) ? 0.0 : 1.0;
+        aw = (f > 0) ? w / f : 0.0;
+        if (valid && fx_out) fx_out[i] = f;
+    }
+    if (!valid) {
+        v0 = 0.0;
+        v1 = 0.0;
+        aw = 0.0;
+    }
+}
+
 // ----------------------------------------------------------- k_shape
 // Pc lineshapes over the event's sub-bins and the per-event likelihood term.
 // FULL: K-matrix members or Pc lineshapes outside the factorised range are
@@ -77,11 +126,12 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                                                  double* __restrict__ awp, double* __restrict__ fx_out,
                                                  Partial* __restrict__ part, int noint, GradLayout G, int NGp,
                                                  double* __restrict__ gpart, int rpb) {
+    const SegDesc sd = sv.seg();
+    if (sv.direct(sd)) return;
     extern __shared__ double lds[];
     Emitter em;
     if constexpr (GRAD) em = Emitter::make(lds, NGp);
     double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
-    const SegDesc sd = sv.seg();
     const int scheme = sd.scheme;
     const int nb = S->scheme_nbins[scheme];
     const double inb = 1.0 / nb;
@@ -196,24 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             }
         }
         double v0, v1, aw;
-        if (MC) {
-            const bool unw = (w <= -1000.0);
-            v0 = unw ? f : f * w;
-            v1 = (noint && !unw) ? fni * w : 0.0;
-            aw = unw ? 1.0 : w;
-            if (valid && fx_out) fx_out[i] = v0;
-        } else {
-            if (f > 0) v0 = log(f) * w;
-            else v0 = (S->flags & PWA_F_NEG_FX_PENALTY) ? -1e12 : 0.0;
-            v1 = (f > 0) ? 0.0 : 1.0;
-            aw = (f > 0) ? w / f : 0.0;
-            if (valid && fx_out) fx_out[i] = f;
-        }
-        if (!valid) {
-            v0 = 0.0;
-            v1 = 0.0;
-            aw = 0.0;
-        }
+        event_terms<MC>(S, f, fni, w, noint, valid, fx_out, i, v0, v1, aw);
         kadd(s0, c0, v0);
         kadd(s1, c1, v1);
         if constexpr (GRAD) {
@@ -293,6 +326,108 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
     }
 }
 
+// ---------------------------------------------------------- k_pcdirect
+// Unsmeared events (lineshapes at the event's own m(J/psi p)) when every Pc
+// lineshape has the factorised form: the Pc chain is contracted once,
+// A = H0 + chain_contract(sum_r L_r V_r), f = |A|^2, with no quadratic form.
+// With GRAD the seed Y = A replaces H0 in the amp planes for k_adj_lam, and
+// the Pc coupling and lineshape-parameter adjoints are emitted here (the
+// seeds U_r = L_r conj(A) are rank one: pc_direct_adjoint).  Writes the
+// block's partial sums and gradient row 0 (the k_shape slots of its blocks).
+template <int NZ, bool MC, bool GRAD>
+__global__ void __launch_bounds__(kBlock, 2) k_pcdirect(const double* __restrict__ ev, int64_t ld, SegView sv,
+                                                    const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                    const C64* __restrict__ cpl, const double* __restrict__ pcs,
+                                                    double* __restrict__ amp, double* __restrict__ awp,
+                                                    double* __restrict__ fx_out, Partial* __restrict__ part,
+                                                    int noint, GradLayout G, int NGp, double* __restrict__ gpart,
+                                                    int rpb) {
+    const SegDesc sd = sv.seg();
+    if (!sv.direct(sd)) return;
+    extern __shared__ double lds[];
+    Emitter em;
+    if constexpr (GRAD) em = Emitter::make(lds, NGp);
+    double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
+    const int64_t n = sd.n;
+    for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
+        const int64_t gi = base + threadIdx.x;
+        const bool valid = gi < n;
+        const int64_t i = sd.begin + (valid ? gi : n - 1);
+        const double w = ld_d(ev, ld, i, E_W);
+        const double mjp = ld_d(ev, ld, i, E_MJPSIP);
+        const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
+        const double cZ = ld_d(ev, ld, i, E_Z_CZ), sZ = ld_d(ev, ld, i, E_Z_SZ);
+        const C64* cpl_i = opaque(cpl);
+        const ChainKin kin = chain_kin(S->P, CH_PC, mjp);
+        C64 A[8];
+        const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
+        {
+            LsEv e;
+            e.key = -1;
+            C64 V[2][3][2];
+#pragma unroll
+            for (int r = 0; r < NZ; ++r) {
+                const LsConst& lc = D->ls[S->nR + r];
+                ls_event_cached(S->P, kin, lc, e);
+                pc_vtable<true>(S, cpl_i, S->nR + r, cZ, sZ, cP, sP, ls_value(lc, kin, e), V);
+            }
+            chain_contract(za, V, A);
+        }
+        double f = 0.0, fpc = 0.0;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            fpc += norm2(A[h]);
+            A[h] = A[h] + C64(amp[(int64_t)(2 * h) * ld + i], amp[(int64_t)(2 * h + 1) * ld + i]);
+            f += norm2(A[h]);
+        }
+        const double fni = (MC && noint) ? pcs[ld + i] + fpc : 0.0;
+        double v0, v1, aw;
+        event_terms<MC>(S, f, fni, w, noint, valid, fx_out, i, v0, v1, aw);
+        kadd(s0, c0, v0);
+        kadd(s1, c1, v1);
+        if constexpr (GRAD) {
+            if (valid) {
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    amp[(int64_t)(2 * h) * ld + i] = A[h].re;
+                    amp[(int64_t)(2 * h + 1) * ld + i] = A[h].im;
+                }
+                awp[i] = aw;
+            }
+            const double a2 = 2.0 * aw;
+            C64 T0[2][3][2];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) A[h] = conj(A[h]);
+            chain_adjoint(za, A, T0);
+            // lineshapes again, now with their derivatives
+            LsEv e;
+            e.key = -1;
+#pragma unroll 1
+            for (int r = 0; r < NZ; ++r) {
+                const LsConst& lc = D->ls[S->nR + r];
+                ls_event_cached(S->P, kin, lc, e);
+                C64 L, dL0, dL1;
+                ls_grad(lc, kin, e, L, dL0, dL1);
+                const C64 P = pc_direct_adjoint(S, cpl_i, S->nR + r, T0, cZ, sZ, cP, sP, L, a2, G, em);
+                em.begin(2);
+                em.put(0, a2 * re_mul(P, dL0));
+                em.put(1, a2 * re_mul(P, dL1));
+                em.commit(2, G.res(S->nR + r));
+            }
+        }
+    }
+    if constexpr (GRAD) {
+        if (em.cnt > 0) em.flush();
+    }
+    block_reduce2(s0, c0, s1, c1);
+    if (threadIdx.x == 0) part[blockIdx.x] = Partial{s0, c0, s1, c1};
+    if constexpr (GRAD) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < G.ng; q += kBlock)
+            gpart[((int64_t)rpb * blockIdx.x) * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+    }
+}
+
 // ----------------------------------------------------------- k_adj_pc
 // Pc coupling adjoint from the seeds U_r that k_shape leaves in the amp
 // planes; every contribution weighted by 2 a.
@@ -304,7 +439,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__
     extern __shared__ double lds[];
     Emitter em = Emitter::make(lds, NGp);
     const SegDesc sd = sv.seg();
-    const int64_t n = sd.n;
+    const int64_t n = sv.direct(sd) ? 0 : sd.n;  // k_pcdirect emitted those events: zero rows
     for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
         const int64_t gi = base + threadIdx.x;
         const bool valid = gi < n;
@@ -330,32 +465,31 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_pc(const double* __restrict__
 
 template <int NZ>
 hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
-    const unsigned eblocks = (unsigned)((a.n + kBlock - 1) / kBlock);
     double* pcs = a.inter;
     double* amp = a.inter + (int64_t)n_pcs(NZ) * a.ld;
     double* awp = amp + (int64_t)n_amp(NZ) * a.ld;
     GradLayout G{a.ncpl, a.nres, a.ng};
     const int NGp = (a.ng + 1) & ~1;
     const size_t lds = a.grad ? grad_lds_bytes(a.ng) : 0;
+    const unsigned sblocks = (unsigned)a.nblocks;
     hipError_t e;
     auto mark = [&](int k) { return a.tev ? hipEventRecord(a.tev[k], st) : hipSuccess; };
-    if ((e = mark(0)) != hipSuccess) return e;
+    auto lds_attr = [&](const void* f) {
+        return a.grad ? hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) : hipSuccess;
+    };
+    if ((e = mark(ST_LAM)) != hipSuccess) return e;
     if ((e = launch_lam(a, pcs, amp, a.grad || NZ > 0, st)) != hipSuccess) return e;
-    if ((e = mark(1)) != hipSuccess) return e;
+    if ((e = mark(ST_PCANG)) != hipSuccess) return e;
     if constexpr (NZ > 0) {
-        if (a.grad) hipLaunchKernelGGL((k_pcang<NZ, true>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.cpl, pcs, amp);
-        else hipLaunchKernelGGL((k_pcang<NZ, false>), dim3(eblocks), dim3(kBlock), 0, st, a.ev, a.ld, a.n, a.S, a.cpl, pcs, amp);
+        const dim3 grid(sblocks, (unsigned)a.passes);
+        if (a.grad) hipLaunchKernelGGL((k_pcang<NZ, true>), grid, dim3(kBlock), 0, st, a.ev, a.ld, a.sv, a.S, a.cpl, pcs, amp);
+        else hipLaunchKernelGGL((k_pcang<NZ, false>), grid, dim3(kBlock), 0, st, a.ev, a.ld, a.sv, a.S, a.cpl, pcs, amp);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if ((e = mark(2)) != hipSuccess) return e;
-    const unsigned sblocks = (unsigned)a.nblocks;
+    if ((e = mark(ST_SHAPE)) != hipSuccess) return e;
 #define PWA_SHAPE(MC, GR, FL)                                                                                      \
     do {                                                                                                           \
-        if (GR) {                                                                                                  \
-            e = hipFuncSetAttribute((const void*)k_shape<NZ, MC, GR, FL>,                                          \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                         \
-            if (e != hipSuccess) return e;                                                                         \
-        }                                                                                                          \
+        if ((e = lds_attr((const void*)k_shape<NZ, MC, GR, FL>)) != hipSuccess) return e;                          \
         hipLaunchKernelGGL((k_shape<NZ, MC, GR, FL>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S,  \
                            a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart, a.rpb);                      \
     } while (0)
@@ -372,18 +506,34 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
     }
 #undef PWA_SHAPE
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = mark(3)) != hipSuccess) return e;
+    if ((e = mark(ST_DIRECT)) != hipSuccess) return e;
+    if constexpr (NZ > 0) {
+        if (a.sv.direct_ok) {
+#define PWA_DIRECT(MC, GR)                                                                                         \
+    do {                                                                                                           \
+        if ((e = lds_attr((const void*)k_pcdirect<NZ, MC, GR>)) != hipSuccess) return e;                          \
+        hipLaunchKernelGGL((k_pcdirect<NZ, MC, GR>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S,   \
+                           a.D, a.cpl, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart, a.rpb);               \
+    } while (0)
+            if (a.mc && a.grad) PWA_DIRECT(true, true);
+            else if (a.mc) PWA_DIRECT(true, false);
+            else if (a.grad) PWA_DIRECT(false, true);
+            else PWA_DIRECT(false, false);
+#undef PWA_DIRECT
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    if ((e = mark(ST_ADJ_LAM)) != hipSuccess) return e;
     if (a.grad) {
         if ((e = launch_adj_lam(a, amp, awp, st)) != hipSuccess) return e;
-        if ((e = mark(4)) != hipSuccess) return e;
+        if ((e = mark(ST_ADJ_PC)) != hipSuccess) return e;
         if constexpr (NZ > 0) {
-            e = hipFuncSetAttribute((const void*)k_adj_pc<NZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_adj_pc<NZ>), dim3((unsigned)a.nblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv,
+            if ((e = lds_attr((const void*)k_adj_pc<NZ>)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_adj_pc<NZ>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv,
                                a.S, amp, awp, G, NGp, a.gpart, a.rpb);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
-        if ((e = mark(5)) != hipSuccess) return e;
+        if ((e = mark(ST_COUNT)) != hipSuccess) return e;
     }
     return hipSuccess;
 }

@@ -230,7 +230,7 @@ class Context:
         check(lib().pwa_amp2(self._h, kind, dptr(p), dptr(out)), self._h, "pwa_amp2")
         return out
 
-    STAGES = ("lam", "pcang", "shape", "adj_lam", "adj_pc")
+    STAGES = ("lam", "pcang", "shape", "direct", "adj_lam", "adj_pc")
 
     def last_launches(self):
         """Per-segment kernel times of the last evaluation: dicts with ms_<stage>

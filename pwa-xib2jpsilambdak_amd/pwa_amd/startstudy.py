@@ -16,9 +16,10 @@ Procedure (all fits within the same parameter limits):
   2. n_plain random starts fitted in the plain mode: mean time t_plain.
      Expected wall-time of the plain procedure = t_plain / p_random.
   3. Accelerated: n_train analytic fits from random starts record every
-     (parameters, FCN, gradient); the MLP is trained on them; 4096 random
-     starts are descended on the learned surface and ranked; the top n_top
-     are refit (analytic gradient) in rank order until one reaches F*.
+     (parameters, FCN, gradient) with the fit's final FCN; the MLP is trained
+     on them; 4096 random starts are ranked by their predicted fit outcome;
+     the top n_top are refit (analytic gradient) in rank order until one
+     reaches F*.
      Wall-time = data fits + training + ranking + refits up to that success.
 Prints one JSON line.
 """
@@ -81,7 +82,7 @@ def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=64
     # 3. accelerated procedure
     t_acc0 = time.perf_counter()
     tr_starts = random_starts(model, n_train, rng)
-    X, F, G = [], [], []
+    X, F, G, E = [], [], [], []
     train_fits = []
     for x0 in tr_starts:
         r = fit(ctx, model, x0, "analytic", lim, max_iter=1000, max_seconds=fit_seconds, keep_trajectory=True)
@@ -91,15 +92,16 @@ def run(config=1, ndata=20000, nmc=100000, n_random=1000, n_plain=12, n_train=64
                 X.append(x)
                 F.append(f)
                 G.append(g)
+                E.append(r.fval)
     t_data = time.perf_counter() - t_acc0
     _log(f"training data: {len(X)} samples from {n_train} fits in {t_data:.1f} s")
     t1 = time.perf_counter()
     prop = Proposer(model.n_float, *lim, seed=seed)
-    loss = prop.train(np.array(X), np.array(F), np.array(G), epochs=epochs)
+    loss = prop.train(np.array(X), np.array(F), np.array(G), np.array(E), epochs=epochs)
     t_train = time.perf_counter() - t1
     _log(f"MLP trained: loss {loss:.4f} in {t_train:.1f} s")
     t2 = time.perf_counter()
-    cand, score = prop.propose(random_starts(model, n_rank, rng), n_top)
+    cand, score = prop.rank(random_starts(model, n_rank, rng), n_top)
     t_rank = time.perf_counter() - t2
     refits = []
     for x0 in cand:
