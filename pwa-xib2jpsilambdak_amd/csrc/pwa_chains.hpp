@@ -377,6 +377,33 @@ __device__ __forceinline__ double pc_form(const C64 L[NZ], const C64 p0[NZ], con
     return f;
 }
 
+// Factors P_r = p0_r + sum_q conj(L_q) M[q][r] (M[r][r] = Md_r, M[r][q] =
+// conj(M[q][r])): the sub-bin intensity is |H0|^2 + pc_value and the
+// lineshape adjoint is d|A|^2 = 2 Re(dL_r P_r).
+template <int NZ>
+__device__ __forceinline__ void pc_pfactors(const C64 L[NZ], const C64 p0[NZ], const double Md[NZ],
+                                            const C64 Mo[NZ][NZ], C64 P[NZ]) {
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) {
+        C64 Pr = p0[r];
+        rmac(Pr, Md[r], conj(L[r]));
+#pragma unroll
+        for (int q = 0; q < NZ; ++q) {
+            if (q < r) Pr = Pr + conj(L[q]) * Mo[q][r];
+            else if (q > r) Pr = Pr + conj(L[q]) * conj(Mo[r][q]);
+        }
+        P[r] = Pr;
+    }
+}
+// 2 Re(sum_r L_r p0_r) + sum_{q r} conj(L_q) L_r M[q][r] = sum_r Re(L_r (p0_r + P_r))
+template <int NZ>
+__device__ __forceinline__ double pc_value(const C64 L[NZ], const C64 p0[NZ], const C64 P[NZ]) {
+    double f = 0.0;
+#pragma unroll
+    for (int r = 0; r < NZ; ++r) f += re_mul(L[r], p0[r] + P[r]);
+    return f;
+}
+
 // --------------------------------------------------------- reduction
 __device__ __forceinline__ void kadd(double& s, double& c, double x) {
     const double t = s + x;

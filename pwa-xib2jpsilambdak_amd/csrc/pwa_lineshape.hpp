@@ -149,6 +149,44 @@ PWA_HD void ls_grad(const LsConst& c, const ChainKin& k, const LsEv& e, C64& v, 
     d1 = w * (c.C[2] * e.G) + dw1 * cg;
 }
 
+// Two sub-bin masses at once: the key test and the kind test are uniform,
+// so both evaluations sit in one branch and their dependency chains
+// interleave.
+PWA_HD void ls_event_cached2(const Masses& P, const ChainKin k[2], const LsConst& c, LsEv e[2]) {
+    const bool vo = c.kind == LS_VERTEX;
+    const int key = vo ? (64 + c.LB) : (c.LR * 8 + c.LB);
+    if (key != e[0].key) {
+        e[0] = ls_event(P, k[0], c.LR, c.LB, vo);
+        e[1] = ls_event(P, k[1], c.LR, c.LB, vo);
+    }
+}
+PWA_HD void ls_grad2(const LsConst& c, const ChainKin k[2], const LsEv e[2], C64 v[2], C64 d0[2], C64 d1[2]) {
+    if (c.kind == LS_VERTEX) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            v[j] = C64(c.C[0] * e[j].GB, 0.0);
+            d0[j] = C64(c.C[1] * e[j].GB, 0.0);
+            d1[j] = C64(c.C[2] * e[j].GB, 0.0);
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double a = c.m2[0] - k[j].s;
+        const double b = c.K[0] * e[j].E;
+        const double inv = rcp_nr(a * a + b * b);
+        const C64 w(a * inv, b * inv);
+        const C64 w2 = w * w;
+        const double cg = c.C[0] * e[j].G;
+        const double t = cg * inv;
+        v[j] = C64(a * t, b * t);
+        const C64 dw0 = w2 * C64(-c.m2[1], c.K[1] * e[j].E);
+        const C64 dw1 = w2 * C64(-c.m2[2], c.K[2] * e[j].E);
+        d0[j] = w * (c.C[1] * e[j].G) + dw0 * cg;
+        d1[j] = w * (c.C[2] * e[j].G) + dw1 * cg;
+    }
+}
+
 // Host: constants of resonance r for this evaluation (kind LS_GENERIC when
 // the branch conditions of the direct evaluation leave the usual ranges).
 // type: PWA_BREITWIGNER / PWA_NONRESONANT / PWA_KMATRIX (vertex term only).

@@ -182,11 +182,57 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             const double sig = S->mres[0] + mjp * (S->mres[1] + mjp * S->mres[2]);
             double acc = 0.0;
             C64 L[NZ];
+            if constexpr (GRAD && !FULL) {
+            // two sub-bins per pass (the second one zero-weighted past the end)
+            for (int b = 0; b < nb; b += 2) {
+                const bool two = b + 1 < nb;
+                ChainKin kin[2];
+                kin[0] = chain_kin(S->P, CH_PC, mjp + S->scheme_z[scheme][b] * sig);
+                kin[1] = chain_kin(S->P, CH_PC, two ? mjp + S->scheme_z[scheme][b + 1] * sig : mjp);
+                LsEv e[2];
+                e[0].key = -1;
+                C64 Lj[2][NZ], dA[2][NZ], dB[2][NZ];
+#pragma unroll
+                for (int r = 0; r < NZ; ++r) {
+                    const LsConst& lc = D->ls[S->nR + r];
+                    ls_event_cached2(S->P, kin, lc, e);
+                    C64 v[2], d0[2], d1[2];
+                    ls_grad2(lc, kin, e, v, d0, d1);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        Lj[j][r] = v[j];
+                        dA[j][r] = d0[j];
+                        dB[j][r] = d1[j];
+                    }
+                }
+                if (!two) {
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) Lj[1][r] = dA[1][r] = dB[1][r] = C64(0.0, 0.0);
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    C64 Pr[NZ];
+                    pc_pfactors<NZ>(Lj[j], p0, Md, Mo, Pr);
+                    acc += pc_value<NZ>(Lj[j], p0, Pr);
+#pragma unroll
+                    for (int r = 0; r < NZ; ++r) {
+                        BW1[r] = BW1[r] + Lj[j][r];
+                        B2d[r] += norm2(Lj[j][r]);
+#pragma unroll
+                        for (int q = 0; q < r; ++q) B2o[q][r] = B2o[q][r] + Lj[j][q] * conj(Lj[j][r]);
+                        gm[r] += re_mul(Pr[r], dA[j][r]);
+                        gw[r] += re_mul(Pr[r], dB[j][r]);
+                    }
+                }
+            }
+            } else
             for (int b = 0; b < nb; ++b) {
                 const double mb = mjp + S->scheme_z[scheme][b] * sig;
+                C64 Pr[NZ];
                 if constexpr (!GRAD) {
                     pc_lineshapes<NZ, FULL>(S, D, mb, L);
-                    acc += pc_form<NZ>(L, p0, Md, Mo);
+                    pc_pfactors<NZ>(L, p0, Md, Mo, Pr);
+                    acc += pc_value<NZ>(L, p0, Pr);
                 } else {
                     C64 Mt[NZ], dL0[NZ], dL1[NZ];
                     double Vx[NZ];
@@ -209,7 +255,8 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
 #pragma unroll
                     for (int r = 0; r < NZ; ++r)
                         pc_ls_grad<FULL>(S, D, S->nR + r, kin, e, mb, Mt[r], L[r], dL0[r], dL1[r], Vx[r]);
-                    acc += pc_form<NZ>(L, p0, Md, Mo);
+                    pc_pfactors<NZ>(L, p0, Md, Mo, Pr);
+                    acc += pc_value<NZ>(L, p0, Pr);
                     C64 Ab[8];
 #pragma unroll
                     for (int r = 0; r < NZ; ++r) {
@@ -217,18 +264,10 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                         B2d[r] += norm2(L[r]);
 #pragma unroll
                         for (int q = 0; q < r; ++q) B2o[q][r] = B2o[q][r] + L[q] * conj(L[r]);
-                        // P_r = p0_r + sum_q conj(L_q) M[q][r]
-                        C64 Pr = p0[r];
-                        rmac(Pr, Md[r], conj(L[r]));
-#pragma unroll
-                        for (int q = 0; q < NZ; ++q) {
-                            if (q < r) Pr = Pr + conj(L[q]) * Mo[q][r];
-                            else if (q > r) Pr = Pr + conj(L[q]) * conj(Mo[r][q]);
-                        }
-                        gm[r] += re_mul(Pr, dL0[r]);
-                        gw[r] += re_mul(Pr, dL1[r]);
+                        gm[r] += re_mul(Pr[r], dL0[r]);
+                        gw[r] += re_mul(Pr[r], dL1[r]);
                         if (ikc >= 0 && S->ik[S->nR + r] == ikc) {
-                            const C64 add = Pr * Vx[r];
+                            const C64 add = Pr[r] * Vx[r];
 #pragma unroll
                             for (int q = 0; q < 8; ++q)
                                 if (q == S->ikm[S->nR + r]) Ab[q] = Ab[q] + add;
