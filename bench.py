@@ -137,30 +137,76 @@ def build_workload(nd, nm, seed=2024):
     return model, p, dcols, mcols, time.time() - t0
 
 
-def cpu_baseline(model, p, dcols, mcols, nd, nm, budget_s=20.0):
-    """The CPU FP64 oracle on tables prepared once, single thread, on a
-    bounded event sample; scaled to the full workload and to NLL + central
-    finite-difference gradient (2P + 1 NLL evaluations)."""
-    import oracle
-    frac = 0.01
-    sd, sm = max(int(nd * frac), 1), max(int(nm * frac), 1)
-    dsub, msub = dcols[:, :sd], mcols[:, :sm]
-    P = oracle.Prepared(dsub, np.ones(sd), models.window_schemes(dsub[1]), msub, np.ones(sm),
-                        models.window_schemes(msub[1]))
+def _cpu_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the GPU
+    box's share), else the visible cores."""
+    try:
+        return max(int(os.environ.get("OMP_NUM_THREADS", "")), 1)
+    except ValueError:
+        return max(os.cpu_count() or 1, 1)
+
+
+def _time_nll(P, desc, params, threads, budget_s, max_evals):
+    P.nll(desc, params, threads=threads)  # first touch of the tables
     t0 = time.time()
     k = 0
     while True:
-        P.nll(model.desc, p)
+        P.nll(desc, params, threads=threads)
         k += 1
-        if time.time() - t0 > budget_s or k >= 50:
+        if time.time() - t0 > budget_s or k >= max_evals:
             break
-    t_sample = (time.time() - t0) / k
-    t_full = t_sample * (nd + nm) / (sd + sm)
+    return (time.time() - t0) / k, k
+
+
+def cpu_baseline(model, p, dcols, mcols, nd, nm, full_c1=True):
+    """The CPU FP64 oracle (its AVX2/FMA build when the host has those) on
+    tables prepared once, timed on a 1 % event sample of this workload with
+    one thread and with all threads, scaled to the full workload and to NLL +
+    central finite-difference gradient (2P + 1 NLL evaluations).  Also: the
+    NLL shift of FP32 storage (the reference's float tables and couplings)
+    on the sample, and (full_c1) the same timings on the complete 5-Lambda*
+    workload of 20k data / 100k MC events."""
+    import oracle
+    fast = oracle.host_has_avx2_fma()
+    nth = _cpu_threads()
+    frac = 0.01
+    sd, sm = max(int(nd * frac), 1), max(int(nm * frac), 1)
+    dsub, msub = dcols[:, :sd], mcols[:, :sm]
+    dsch, msch = models.window_schemes(dsub[1]), models.window_schemes(msub[1])
+    P = oracle.Prepared(dsub, np.ones(sd), dsch, msub, np.ones(sm), msch, fast=fast)
+    t1, k1 = _time_nll(P, model.desc, p, 1, 8.0, 30)
+    tn, kn = _time_nll(P, model.desc, p, nth, 6.0, 60)
     nfl = int(model.float_mask.sum())
-    return {"value": 1.0 / (t_full * (2 * nfl + 1)), "unit": "NLL+grad evals/s", "cores": 1, "kind": "port",
-            "sample": f"oracle NLL on {sd} data + {sm} MC events (1% of the workload), {k} evaluations, "
-                      f"{t_sample:.3f} s each; scaled x{(nd + nm) / (sd + sm):.0f} to the full sample and "
-                      f"x{2 * nfl + 1} (central finite differences over {nfl} parameters)"}
+    scale = (nd + nm) / (sd + sm)
+    v64 = P.nll(model.desc, p, threads=nth)
+    del P
+    P32 = oracle.Prepared(dsub, np.ones(sd), dsch, msub, np.ones(sm), msch, fp32_storage=True, fast=fast)
+    v32 = P32.nll(model.desc, p, threads=nth)
+    del P32
+    out = {"value": 1.0 / (tn * scale * (2 * nfl + 1)), "unit": "NLL+grad evals/s", "cores": nth, "kind": "port",
+           "sample": f"oracle ({'AVX2/FMA build' if fast else 'portable build'}) NLL on {sd} data + {sm} MC events "
+                     f"(1% of the workload), {kn} evaluations with {nth} threads at {tn:.4f} s and {k1} with 1 thread "
+                     f"at {t1:.4f} s; scaled x{scale:.0f} to the full sample and x{2 * nfl + 1} (central finite "
+                     f"differences over {nfl} parameters)",
+           "single_thread": {"value": 1.0 / (t1 * scale * (2 * nfl + 1)), "cores": 1},
+           "nll_only_evals_per_s": {"threads": 1.0 / (tn * scale), "single_thread": 1.0 / (t1 * scale)},
+           "fp32_storage_nll_shift": {"sample": v32 - v64, "note": "NLL(FP32-stored tables, couplings, parameters) - "
+                                      "NLL(FP64) on the 1% sample; arithmetic FP64"}}
+    if full_c1:
+        m1 = models.config_small(1)
+        mc1 = generate_phsp(100_000, 77)
+        dc1 = toys.accept_reject(m1, 20_000, 78)
+        z1, zm = np.zeros(dc1.shape[1], np.int32), np.zeros(mc1.shape[1], np.int32)
+        Q = oracle.Prepared(dc1, np.ones(dc1.shape[1]), z1, mc1, np.ones(100_000), zm, fast=fast)
+        q1, _ = _time_nll(Q, m1.desc, m1.params, 1, 4.0, 10)
+        qn, _ = _time_nll(Q, m1.desc, m1.params, nth, 3.0, 30)
+        n1 = int(m1.float_mask.sum())
+        out["c1_full"] = {"workload": "5 Lambda* BW, 20k data / 100k MC (complete)",
+                          "nll_evals_per_s": {"threads": 1.0 / qn, "single_thread": 1.0 / q1},
+                          "nll_fd_grad_evals_per_s": {"threads": 1.0 / (qn * (2 * n1 + 1)),
+                                                      "single_thread": 1.0 / (q1 * (2 * n1 + 1))},
+                          "n_float_params": n1}
+    return out
 
 
 def main():
@@ -266,7 +312,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded phase-space MC; data by accept-reject from the model)",
         "config": {"workload": "nominal: 300k data / 3M MC, 14 Lambda* + 3 Pc BW + 1 Zcs, 40-sub-bin smearing",
-                   "n_data": nd, "n_mc": nm, "n_float_params": nfl, "smeared_fraction_mc": round(frac_smeared, 4)},
+                   "n_data": nd, "n_mc": nm, "n_float_params": nfl, "smeared_fraction_mc": round(frac_smeared, 4),
+                   "n_smeared": {"data": n_smeared[abi.PWA_DATA], "mc": n_smeared[abi.PWA_MC]}},
         "plain_fd": {"forward_ms": round(t_fwd * 1e3, 4), "evals_per_s": fd_rate,
                      "speedup": round((1.0 / t_grad) / fd_rate, 2)},
         "roofline": roof,

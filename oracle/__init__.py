@@ -12,7 +12,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+FAST_LIB_PATH = os.path.join(_HERE, "liboracle_fast.so")  # AVX2/FMA build, for the CPU baseline
+_libs = {}
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
@@ -21,12 +22,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
+def host_has_avx2_fma():
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((ln for ln in f if ln.startswith("flags")), "").split()
+        return "avx2" in flags and "fma" in flags
+    except OSError:
+        return False
+
+
+def lib(path=LIB_PATH):
+    _lib = _libs.get(path)
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp = C.c_void_p
         sig = {
             "orc_dfun": (C.c_double, [C.c_double] * 4),
@@ -55,12 +65,15 @@ def lib():
             "orc_prepare": (C.c_void_p, [C.c_int64, _dp, C.c_int64, _dp, _ip, C.c_int64, _dp, C.c_int64, _dp, _ip]),
             "orc_free": (None, [C.c_void_p]),
             "orc_nll_prepared": (C.c_int, [C.c_void_p, vp, _dp, _dp]),
+            "orc_prepare_ex": (C.c_void_p, [C.c_int64, _dp, C.c_int64, _dp, _ip, C.c_int64, _dp, C.c_int64, _dp,
+                                            _ip, C.c_int]),
+            "orc_nll_prepared_ex": (C.c_int, [C.c_void_p, vp, _dp, C.c_int, C.c_int, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
+        _lib = _libs[path] = L
     return _lib
 
 
@@ -155,23 +168,28 @@ def fcn(nll_value, nll_shift, alpha_value, params, idx, mu1, mu2, s1, s2):
 
 
 class Prepared:
-    """Oracle NLL on tables filled once (the CPU-baseline form)."""
+    """Oracle NLL on tables filled once (the CPU-baseline form).
+    fp32_storage: tables, event masses, weights, couplings and resonance
+    parameters rounded to FP32 as the reference stores them (arithmetic FP64);
+    fast: the AVX2/FMA build of the same source."""
 
-    def __init__(self, dcols, sw, dscheme, mcols, wmc, mscheme):
+    def __init__(self, dcols, sw, dscheme, mcols, wmc, mscheme, fp32_storage=False, fast=False):
+        self._lib = lib(FAST_LIB_PATH if fast else LIB_PATH)
+        self.fp32 = 1 if fp32_storage else 0
         self._keep = [_d(dcols), _d(sw), _i(dscheme), _d(mcols), _d(wmc), _i(mscheme)]
         dc, s, ds, mc, w, ms = self._keep
-        self.h = lib().orc_prepare(dc.shape[1], dp(dc), dc.shape[1], dp(s), ip(ds), mc.shape[1], dp(mc), mc.shape[1],
-                                   dp(w), ip(ms))
+        self.h = self._lib.orc_prepare_ex(dc.shape[1], dp(dc), dc.shape[1], dp(s), ip(ds), mc.shape[1], dp(mc),
+                                          mc.shape[1], dp(w), ip(ms), self.fp32)
 
-    def nll(self, desc, params):
+    def nll(self, desc, params, threads=1):
         out = np.zeros(1)
         p = _d(params)
-        if lib().orc_nll_prepared(self.h, dref(desc), dp(p), dp(out)):
-            raise RuntimeError("orc_nll_prepared failed")
+        if self._lib.orc_nll_prepared_ex(self.h, dref(desc), dp(p), int(threads), self.fp32, dp(out)):
+            raise RuntimeError("orc_nll_prepared_ex failed")
         return float(out[0])
 
     def __del__(self):
         try:
-            lib().orc_free(self.h)
+            self._lib.orc_free(self.h)
         except Exception:
             pass

@@ -23,6 +23,9 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -1262,55 +1265,115 @@ int orc_nll(const pwa_model_desc* d, const double* par, int64_t nd, const double
     return 0;
 }
 
-void* orc_prepare(int64_t nd, const double* dcols, int64_t dld, const double* sw, const int32_t* dscheme, int64_t nm,
-                  const double* mcols, int64_t mld, const double* wmc, const int32_t* mscheme) {
+// fp32_storage: the tables, event masses and weights rounded to FP32 as the
+// reference stores them (float planes, src/RooDalitzAmplitude.cxx:557-770);
+// the arithmetic stays FP64.
+void* orc_prepare_ex(int64_t nd, const double* dcols, int64_t dld, const double* sw, const int32_t* dscheme,
+                     int64_t nm, const double* mcols, int64_t mld, const double* wmc, const int32_t* mscheme,
+                     int fp32_storage) {
     Prepared* h = new Prepared();
     double ev[PWA_NCOL];
     h->dz.resize(nd);
     h->mz.resize(nm);
+    auto round_tables = [](Dlz& z) {
+        double* p = reinterpret_cast<double*>(&z);
+        for (size_t k = 0; k < sizeof(Dlz) / sizeof(double); ++k) p[k] = (double)(float)p[k];
+    };
+    auto st = [&](double x) { return fp32_storage ? (double)(float)x : x; };
     for (int64_t i = 0; i < nd; ++i) {
         gather(dcols, dld, i, ev);
         filldlz(h->dz[i], ev);
-        h->sw.push_back(sw[i]);
+        if (fp32_storage) round_tables(h->dz[i]);
+        h->sw.push_back(st(sw[i]));
         h->ds.push_back(dscheme ? dscheme[i] : 0);
     }
     for (int64_t i = 0; i < nm; ++i) {
         gather(mcols, mld, i, ev);
         filldlz(h->mz[i], ev);
-        h->w.push_back(wmc[i]);
+        if (fp32_storage) round_tables(h->mz[i]);
+        h->w.push_back(st(wmc[i]));
         h->ms.push_back(mscheme ? mscheme[i] : 0);
     }
     return h;
 }
 
+void* orc_prepare(int64_t nd, const double* dcols, int64_t dld, const double* sw, const int32_t* dscheme, int64_t nm,
+                  const double* mcols, int64_t mld, const double* wmc, const int32_t* mscheme) {
+    return orc_prepare_ex(nd, dcols, dld, sw, dscheme, nm, mcols, mld, wmc, mscheme, 0);
+}
+
 void orc_free(void* h) { delete static_cast<Prepared*>(h); }
 
-// The NLL of orc_nll on prepared tables.
-int orc_nll_prepared(void* hp, const pwa_model_desc* d, const double* par, double* nll) {
+// The NLL of orc_nll on prepared tables, over nthreads OpenMP threads (static
+// event blocks, partial sums combined in thread order, so a given thread
+// count is deterministic; 1 thread is the plain sequential sum).
+// fp32_storage: couplings, resonance parameters and K-matrix blocks rounded to
+// FP32 as the reference uploads them (_vamp, h_paraList, src/RooDalitzAmplitude.cxx:3141-3210).
+int orc_nll_prepared_ex(void* hp, const pwa_model_desc* d, const double* par, int nthreads, int fp32_storage,
+                        double* nll) {
     Model M;
     if (!hp || !model_from(d, M)) return -1;
     const Prepared& h = *static_cast<Prepared*>(hp);
     AmpParams ap;
     build_amp(M, par, ap);
-    Phys P = phys_from(d);
-    double sumint = 0, sumnoint = 0, sumW = 0, sumW_data = 0, nll_data = 0;
-    for (size_t i = 0; i < h.mz.size(); ++i) {
-        const double a2 = table_amp2(M, ap, P, h.mz[i], h.ms[i], true);
-        const double w = h.w[i];
-        sumint += (w <= -1000.) ? a2 : a2 * w;
-        sumW += w;
-        if ((d->flags & PWA_F_NOINT_PENALTY) && !(w <= -1000.)) sumnoint += table_amp2(M, ap, P, h.mz[i], 0, false) * w;
+    if (fp32_storage) {
+        auto r = [](double x) { return (double)(float)x; };
+        for (auto& v : ap.vamp) v = cd(r(v.real()), r(v.imag()));
+        for (auto& v : ap.m0) v = r(v);
+        for (auto& v : ap.g0) v = r(v);
+        for (auto& row : ap.km)
+            for (auto& v : row) v = r(v);
     }
+    Phys P = phys_from(d);
+    const int nt = std::max(nthreads, 1);
+    std::vector<double> p_int(nt, 0.0), p_noint(nt, 0.0), p_w(nt, 0.0), p_wd(nt, 0.0), p_nll(nt, 0.0);
+    const bool noint = (d->flags & PWA_F_NOINT_PENALTY) != 0;
     const bool negpen = (d->flags & PWA_F_NEG_FX_PENALTY) != 0;
-    for (size_t i = 0; i < h.dz.size(); ++i) {
-        const double fx = table_amp2(M, ap, P, h.dz[i], h.ds[i], true);
-        sumW_data += h.sw[i];
-        nll_data += (fx > 0) ? std::log(fx) * h.sw[i] : (negpen ? -1e12 : 0.0);
+    const int64_t nm = (int64_t)h.mz.size(), nd = (int64_t)h.dz.size();
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+        const int t = omp_get_thread_num();
+        const int T = omp_get_num_threads();
+#else
+        const int t = 0;
+        const int T = 1;
+#endif
+        double si = 0, sn = 0, sw = 0, swd = 0, sl = 0;
+        for (int64_t i = nm * t / T; i < nm * (t + 1) / T; ++i) {
+            const double a2 = table_amp2(M, ap, P, h.mz[i], h.ms[i], true);
+            const double w = h.w[i];
+            si += (w <= -1000.) ? a2 : a2 * w;
+            sw += w;
+            if (noint && !(w <= -1000.)) sn += table_amp2(M, ap, P, h.mz[i], 0, false) * w;
+        }
+        for (int64_t i = nd * t / T; i < nd * (t + 1) / T; ++i) {
+            const double fx = table_amp2(M, ap, P, h.dz[i], h.ds[i], true);
+            swd += h.sw[i];
+            sl += (fx > 0) ? std::log(fx) * h.sw[i] : (negpen ? -1e12 : 0.0);
+        }
+        p_int[t] = si;
+        p_noint[t] = sn;
+        p_w[t] = sw;
+        p_wd[t] = swd;
+        p_nll[t] = sl;
+    }
+    double sumint = 0, sumnoint = 0, sumW = 0, sumW_data = 0, nll_data = 0;
+    for (int t = 0; t < nt; ++t) {
+        sumint += p_int[t];
+        sumnoint += p_noint[t];
+        sumW += p_w[t];
+        sumW_data += p_wd[t];
+        nll_data += p_nll[t];
     }
     double v = d->nll_shift + sumW_data * std::log(sumint / sumW) - nll_data;
-    if ((d->flags & PWA_F_NOINT_PENALTY) && std::fabs(sumnoint / sumint - 1.0) > 0.3) v += 1000.0;
+    if (noint && std::fabs(sumnoint / sumint - 1.0) > 0.3) v += 1000.0;
     *nll = v;
     return 0;
+}
+
+int orc_nll_prepared(void* hp, const pwa_model_desc* d, const double* par, double* nll) {
+    return orc_nll_prepared_ex(hp, d, par, 1, 0, nll);
 }
 
 // Event classification of the reference for sorted data

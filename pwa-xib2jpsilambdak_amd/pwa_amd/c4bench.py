@@ -1,0 +1,117 @@
+"""Toy x parameter-point batch throughput: NLL + gradient of many toy data
+sets at several parameter points against one shared MC sample, in one pass
+over all events per point (pwa_upload_toys / pwa_eval_batch).
+
+  python -m pwa_amd.c4bench [--toys 256] [--points 8] [--nmc 3000000]
+
+Workload: the K-matrix model (models.config_kmatrix: 14 Lambda*, a
+4-channel K-matrix Pc sector with 2 poles, 1 Zcs, 40-sub-bin smearing);
+toy sizes N_i log-uniform in [1e4, 2e5] (per-toy seeds); toy events drawn
+with replacement from an accept-reject pool of the model (`--pool` events,
+generated on the GPU), so the toys follow the model without generating
+~1.6e7 events by accept-reject; 3M phase-space MC shared by all toys;
+points = the generating parameters and random 2 % variations of the floating
+ones.  The comparison is the same evaluations made one toy at a time (each
+toy uploaded alone with the shared MC, pwa_eval_grad at every point), timed
+on `--seq-toys` toys and scaled by their share of the events.  Prints one
+JSON line.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import abi, models, toys
+from ._lib import Context, generate_phsp
+
+
+def _log(msg):
+    print(f"[c4bench] {msg}", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--toys", type=int, default=256)
+    ap.add_argument("--points", type=int, default=8)
+    ap.add_argument("--nmc", type=int, default=3_000_000)
+    ap.add_argument("--pool", type=int, default=1_000_000)
+    ap.add_argument("--seq-toys", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=404)
+    a = ap.parse_args()
+
+    model = models.config_kmatrix()
+    t0 = time.perf_counter()
+    mcols = generate_phsp(a.nmc, a.seed)
+    msch = models.window_schemes(mcols[abi.COL["mjpsiL"]])
+    pool = toys.accept_reject(model, a.pool, a.seed + 1)
+    rng = np.random.default_rng(a.seed + 2)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e5), size=a.toys)).astype(np.int64)
+    idx = np.concatenate([np.random.default_rng(a.seed + 100 + t).integers(0, pool.shape[1], n)
+                          for t, n in enumerate(sizes)])
+    cols = np.ascontiguousarray(pool[:, idx])
+    del pool
+    dsch = models.window_schemes(cols[abi.COL["mjpsiL"]])
+    w = np.ones(cols.shape[1])
+    pts = np.tile(model.params, (a.points, 1))
+    fl = np.flatnonzero(model.float_mask)
+    pts[1:, fl] *= 1.0 + 0.02 * np.random.default_rng(a.seed + 3).standard_normal((a.points - 1, len(fl)))
+    n_tot = int(sizes.sum())
+    _log(f"setup {time.perf_counter() - t0:.1f} s: {a.toys} toys, {n_tot} data events, {a.nmc} MC, "
+         f"{a.points} points, {len(fl)} floating parameters")
+
+    # batched: all toys at every point, one pass over all events per point
+    with Context(model.desc) as c:
+        c.upload_toys(sizes, cols, w, dsch)
+        c.upload(abi.PWA_MC, mcols, np.ones(a.nmc), msch)
+        c.eval_batch(pts[:1], grad=True)
+        t1 = time.perf_counter()
+        nll, g = c.eval_batch(pts, grad=True)
+        t_batch = time.perf_counter() - t1
+    _log(f"batch: {t_batch:.3f} s for {a.toys * a.points} toy-point NLL + gradient evaluations")
+
+    # one toy at a time, same points
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    t_seq, n_seq, worst = 0.0, 0, 0.0
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_MC, mcols, np.ones(a.nmc), msch)
+        for t in range(a.seq_toys):
+            sl = slice(off[t], off[t + 1])
+            c.upload(abi.PWA_DATA, cols[:, sl], w[sl], dsch[sl])
+            c.eval_grad(pts[0])
+            t1 = time.perf_counter()
+            for p in range(a.points):
+                v, gv = c.eval_grad(pts[p])
+                worst = max(worst, abs(v - nll[p, t]) / max(abs(v), 1.0))
+            t_seq += time.perf_counter() - t1
+            n_seq += int(sizes[t])
+    # the MC pass is per evaluation, the data pass per event: scale the data part
+    # by the event share and the MC part by the toy count
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_MC, mcols, np.ones(a.nmc), msch)
+        c.upload(abi.PWA_DATA, cols[:, :1], w[:1], dsch[:1])
+        c.eval_grad(pts[0])
+        t1 = time.perf_counter()
+        for p in range(a.points):
+            c.eval_grad(pts[p])
+        t_mc = time.perf_counter() - t1  # MC pass (plus one event) at every point
+    t_data_per_event = max(t_seq - a.seq_toys * t_mc, 0.0) / n_seq
+    t_seq_all = a.toys * t_mc + n_tot * t_data_per_event
+    out = {
+        "workload": "C4: config_kmatrix, toys N_i log-uniform in [1e4, 2e5] resampled from an accept-reject pool, "
+                    "shared phase-space MC",
+        "toys": a.toys, "points": a.points, "data_events": n_tot, "mc_events": a.nmc, "pool_events": a.pool,
+        "n_float_params": int(len(fl)),
+        "batch_s": round(t_batch, 4),
+        "toy_point_evals_per_s": round(a.toys * a.points / t_batch, 1),
+        "sequential": {"toys_timed": a.seq_toys, "events_timed": n_seq, "seconds_timed": round(t_seq, 4),
+                       "mc_pass_s_per_point": round(t_mc / a.points, 5),
+                       "estimated_s_all_toys": round(t_seq_all, 3)},
+        "speedup_vs_sequential": round(t_seq_all / t_batch, 2),
+        "max_rel_nll_diff_batch_vs_single": worst,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
