@@ -204,8 +204,16 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
     for (int g = 0; g < 5; ++g) {
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
-        double dl[2][6];
-        lambda_dl(g, cL, sL, dl);
+        // the group's Wigner factors folded into Q once: dq = Q dl
+        C64 dq[2][6];
+        {
+            double dl[2][6];
+            lambda_dl(g, cL, sL, dl);
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int m = 0; m < 6; ++m) dq[k][m] = Q[k][m] * dl[k][m];
+        }
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
@@ -221,11 +229,10 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
                     if (m >= nm) continue;
-                    const C64 dq = Q[k][m] * dl[k][m];
-                    const C64 z = Lv * dq;
+                    const C64 z = Lv * dq[k][m];
                     em.put(2 * m, z.re);
-                    em.put(2 * m + 1, -z.im);
-                    cmac(R, c[k * nm + m], dq);
+                    em.put(2 * m + 1, z.im);
+                    cmac(R, c[k * nm + m], dq[k][m]);
                 }
                 em.commit(2 * nm, base + 2 * k * nm);
             }
@@ -304,7 +311,7 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
                         const C64 t = T[k][ps][iz] * w;
                         const C64 z = Lv * t;
                         em.put(2 * (n - n0), z.re);
-                        em.put(2 * (n - n0) + 1, -z.im);
+                        em.put(2 * (n - n0) + 1, z.im);
                         cmac(R, c[n], t);
                         ++n;
                     }
@@ -342,7 +349,7 @@ __device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict_
                     const double w = a2 * dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
                     const C64 z = T[k][ps][iz] * w;
                     em.put(2 * (n - n0), z.re);
-                    em.put(2 * (n - n0) + 1, -z.im);
+                    em.put(2 * (n - n0) + 1, z.im);
                     ++n;
                 }
             em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
@@ -381,7 +388,7 @@ __device__ __forceinline__ C64 pc_direct_adjoint(const DevStatic* __restrict__ S
                     cmac(P, t, c[n]);
                     const C64 z = La * t;
                     em.put(2 * (n - n0), z.re);
-                    em.put(2 * (n - n0) + 1, -z.im);
+                    em.put(2 * (n - n0) + 1, z.im);
                     ++n;
                 }
             em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));

@@ -610,7 +610,8 @@ void map_gradient(const pwa_ctx* c, const double* params, const std::vector<doub
     pwa_fit_param_count(&d, &np);
     std::fill(grad, grad + np, 0.0);
     std::vector<cplx> gc(c->ncpl);
-    for (int j = 0; j < c->ncpl; ++j) gc[j] = cplx(gk[2 * j], gk[2 * j + 1]);
+    // the kernels emit (Re, Im) of a (dF/dc*) per coupling c; dF/dRe c = Re, dF/dIm c = -Im
+    for (int j = 0; j < c->ncpl; ++j) gc[j] = cplx(gk[2 * j], -gk[2 * j + 1]);
     c->cmap.chain(params, gc.data(), grad);
     const int o_res = 2 * c->ncpl;
     const int o_km = o_res + 2 * c->nres;
