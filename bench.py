@@ -124,8 +124,16 @@ def traffic_per_eval(launches, n_smeared):
     return round(total)
 
 
-def build_workload(nd, nm, seed=2024):
-    model = models.config_nominal(2)
+WORKLOADS = {
+    "nominal": ("nominal: 300k data / 3M MC, 14 Lambda* + 3 Pc BW + 1 Zcs, 40-sub-bin smearing",
+                lambda: models.config_nominal(2)),
+    "kmatrix": ("kmatrix: 300k data / 3M MC, 14 Lambda* + 4-channel K-matrix Pc (2 poles) + 1 Zcs, 40-sub-bin smearing",
+                lambda: models.config_kmatrix(3)),
+}
+
+
+def build_workload(nd, nm, seed=2024, name="nominal"):
+    model = WORKLOADS[name][1]()
     t0 = time.time()
     mcols = generate_phsp(nm, seed)
     dcols = toys.accept_reject(model, nd, seed + 1)
@@ -217,6 +225,8 @@ def main():
     ap.add_argument("--ndata", type=int, default=300_000)
     ap.add_argument("--nmc", type=int, default=3_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--model", choices=sorted(WORKLOADS), default="nominal",
+                    help="workload: the metric's nominal model (default) or its K-matrix variant")
     ap.add_argument("--fd-evals", type=int, default=20, help="forward evaluations timed for the FD baseline")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
@@ -236,7 +246,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    model, p, dcols, mcols, t_setup = build_workload(args.ndata, args.nmc)
+    model, p, dcols, mcols, t_setup = build_workload(args.ndata, args.nmc, name=args.model)
     nd, nm = dcols.shape[1], mcols.shape[1]
     ctx = Context(model.desc)
     ds, ms = models.window_schemes(dcols[1]), models.window_schemes(mcols[1])
@@ -311,7 +321,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded phase-space MC; data by accept-reject from the model)",
-        "config": {"workload": "nominal: 300k data / 3M MC, 14 Lambda* + 3 Pc BW + 1 Zcs, 40-sub-bin smearing",
+        "config": {"workload": WORKLOADS[args.model][0] if (nd, nm) == (300_000, 3_000_000)
+                   else f"{args.model}: {nd} data / {nm} MC",
                    "n_data": nd, "n_mc": nm, "n_float_params": nfl, "smeared_fraction_mc": round(frac_smeared, 4),
                    "n_smeared": {"data": n_smeared[abi.PWA_DATA], "mc": n_smeared[abi.PWA_MC]}},
         "plain_fd": {"forward_ms": round(t_fwd * 1e3, 4), "evals_per_s": fd_rate,

@@ -79,6 +79,12 @@ struct DevDyn {
     int32_t km_npole[PWA_KM_NCLASS];
     int32_t km_pole_res[PWA_KM_NCLASS][PWA_KM_NPOLE];
     int32_t km_pole_shift[PWA_KM_NCLASS][PWA_KM_NPOLE];
+    // per pole, functions of the parameters only (km_pole_terms): the squared
+    // pole mass after the threshold shift and its derivatives with respect
+    // to the fitted pole mass and to the coupling of the shift channel
+    double km_m2s[PWA_KM_NCLASS][PWA_KM_NPOLE];
+    double km_dm2m[PWA_KM_NCLASS][PWA_KM_NPOLE];
+    double km_dm2g[PWA_KM_NCLASS][PWA_KM_NPOLE];
 };
 
 // Per-block reduction record: compensated sums (hi, lo).

@@ -15,6 +15,7 @@
 #include "pwa_amp.h"
 #include "pwa_couplings.hpp"
 #include "pwa_device.hpp"
+#include "pwa_kmatrix.hpp"
 
 using namespace pwa;
 
@@ -92,6 +93,7 @@ struct pwa_ctx {
     hipEvent_t le[kMaxLaunch][kStageEvents] = {};  // per sample launch: around each stage kernel
     int nlaunch = 0;
     int32_t l_kind[kMaxLaunch] = {}, l_scheme[kMaxLaunch] = {}, l_grad[kMaxLaunch] = {};
+    uint32_t l_ran[kMaxLaunch] = {};  // bit k: stage kernel k was launched
     int64_t l_n[kMaxLaunch] = {};
     float l_ms[kMaxLaunch] = {};         // whole sample pipeline
     float l_stage[kMaxLaunch][ST_COUNT] = {};  // per stage kernel (enum Stage)
@@ -269,6 +271,7 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
             }
         }
     }
+    km_pole_terms(c.hs, D);
     return ngen;
 }
 
@@ -306,7 +309,8 @@ void collect_times(pwa_ctx* c) {
         c->l_ms[q] = (hipEventElapsedTime(&t, c->le[q][0], c->le[q][last]) == hipSuccess) ? t : -1.f;
         for (int k = 0; k < ST_COUNT; ++k) {
             t = 0.f;
-            c->l_stage[q][k] = (k < last && hipEventElapsedTime(&t, c->le[q][k], c->le[q][k + 1]) == hipSuccess) ? t : 0.f;
+            const bool ran = k < last && ((c->l_ran[q] >> k) & 1u);
+            c->l_stage[q][k] = (ran && hipEventElapsedTime(&t, c->le[q][k], c->le[q][k + 1]) == hipSuccess) ? t : 0.f;
         }
     }
 }
@@ -442,6 +446,10 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
             c->l_scheme[li] = s.segs.size() == 1 ? s.segs[0].scheme : -1;
             c->l_n[li] = s.n;
             c->l_grad[li] = grad ? 1 : 0;
+            const int nz = c->desc.n_pc;
+            c->l_ran[li] = (1u << ST_LAM) | (nz > 0 ? (1u << ST_PCANG) : 0u) | (1u << ST_SHAPE) |
+                           ((nz > 0 && a.sv.direct_ok) ? (1u << ST_DIRECT) : 0u) |
+                           (grad ? ((1u << ST_ADJ_LAM) | (nz > 0 ? (1u << ST_ADJ_PC) : 0u)) : 0u);
         }
         HIPCHK(c, run_stages(c->desc.n_pc, a, c->st));
     }

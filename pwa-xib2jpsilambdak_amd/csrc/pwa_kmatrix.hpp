@@ -41,17 +41,7 @@ static PWA_HD void kmatrix(const DevStatic* __restrict__ S, const DevDyn* __rest
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         if (p >= np) continue;
-        const int r = D->km_pole_res[ik][p];
-        const double mfit = D->m0[r];
-        const int ch = D->km_pole_shift[ik][p];
-        double m02 = mfit * mfit;
-        if (ch > 0) {
-            const C64 rc = rho_cmass(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch]);
-            const double gc = g[p * 4 + ch];
-            const double m0 = sqrt(m02 - gc * gc * rc.im);
-            m02 = m0 * m0;
-        }
-        double d = m02 - s;
+        double d = D->km_m2s[ik][p] - s;
         if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) d = 1e-3;
         den[p] = d;
     }
@@ -158,8 +148,6 @@ struct KMState {
     C64 T[4];
     double na;
     int np;
-    double mfit[4], rcim[4], drcim[4];  // pole mass, Im rho_shift, d Im rho_shift / d mfit
-    int ch[4];
 };
 
 // rho for the complex-mass shift channel and d(Im rho)/dM2 (branch as in csqrt_branch)
@@ -194,27 +182,8 @@ PWA_HD void kmatrix_state(const DevStatic* __restrict__ S, const DevDyn* __restr
     for (int p = 0; p < 4; ++p) {
         st.den[p] = 1.0;
         st.clamped[p] = true;
-        st.ch[p] = 0;
-        st.mfit[p] = 0.0;
-        st.rcim[p] = 0.0;
-        st.drcim[p] = 0.0;
         if (p >= np) continue;
-        const int r = D->km_pole_res[ik][p];
-        const double mfit = D->m0[r];
-        const int ch = D->km_pole_shift[ik][p];
-        st.mfit[p] = mfit;
-        st.ch[p] = ch;
-        double m02 = mfit * mfit;
-        if (ch > 0) {
-            double im, dim;
-            rho_cmass_d(m02, S->km_shift_ma[ch], S->km_shift_mb[ch], S->km_shift_hw[ch], im, dim);
-            const double gc = g[p * 4 + ch];
-            const double m0 = sqrt(m02 - gc * gc * im);
-            st.rcim[p] = im;
-            st.drcim[p] = dim * 2.0 * mfit;
-            m02 = m0 * m0;
-        }
-        double d = m02 - s;
+        double d = D->km_m2s[ik][p] - s;
         bool cl = false;
         if ((S->flags & PWA_F_KM_CLAMP) && fabs(d) < 1e-3) {
             d = 1e-3;
@@ -402,15 +371,39 @@ PWA_HD void kmatrix_adjoint(const DevDyn* __restrict__ D, int ik, const KMState&
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         if (p >= np) continue;
-        const int ch = st.ch[p];
-        if (ch > 0) {
-            const double gc = g[p * 4 + ch];
-            mb[p] += denb[p] * (2.0 * st.mfit[p] - gc * gc * st.drcim[p]);
-            gb[p * 4 + ch] += denb[p] * (-2.0 * gc * st.rcim[p]);
-        } else {
-            mb[p] += denb[p] * 2.0 * st.mfit[p];
-        }
+        mb[p] += denb[p] * D->km_dm2m[ik][p];
+        const int ch = D->km_pole_shift[ik][p];
+        if (ch > 0) gb[p * 4 + ch] += denb[p] * D->km_dm2g[ik][p];
     }
+}
+
+// Host: the per-pole terms of every class from the pole masses and couplings
+// (m0^2 shifted by sqrt(m0^2 - g_c^2 Im rho_c(m0^2)) for a threshold-shift
+// channel c, src/cu_DPFPropogator.cu:695-728), with their derivatives.
+inline void km_pole_terms(const DevStatic& S, DevDyn& D) {
+    for (int ik = 0; ik < PWA_KM_NCLASS; ++ik)
+        for (int p = 0; p < PWA_KM_NPOLE; ++p) {
+            D.km_m2s[ik][p] = 0.0;
+            D.km_dm2m[ik][p] = 0.0;
+            D.km_dm2g[ik][p] = 0.0;
+            if (p >= D.km_npole[ik]) continue;
+            const double mfit = D.m0[D.km_pole_res[ik][p]];
+            const int ch = D.km_pole_shift[ik][p];
+            double m02 = mfit * mfit;
+            double dm = 2.0 * mfit, dg = 0.0;
+            if (ch > 0) {
+                double im, dim;
+                rho_cmass_d(m02, S.km_shift_ma[ch], S.km_shift_mb[ch], S.km_shift_hw[ch], im, dim);
+                const double gc = D.km[ik][p * 4 + ch];
+                const double m0 = sqrt(m02 - gc * gc * im);
+                m02 = m0 * m0;
+                dm = 2.0 * mfit - gc * gc * dim * 2.0 * mfit;
+                dg = -2.0 * gc * im;
+            }
+            D.km_m2s[ik][p] = m02;
+            D.km_dm2m[ik][p] = dm;
+            D.km_dm2g[ik][p] = dg;
+        }
 }
 
 

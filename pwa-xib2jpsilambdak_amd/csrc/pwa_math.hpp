@@ -168,6 +168,11 @@ template <class T> PWA_HD Cx<T> cinv(const Cx<T>& a) {
     T s = 1.0 / (a.re * a.re + a.im * a.im);
     return Cx<T>(a.re * s, (0.0 - a.im) * s);
 }
+// FP64: the refined hardware reciprocal (the IEEE value for normal arguments)
+PWA_HD C64 cinv(const C64& a) {
+    const double s = rcp_nr(a.re * a.re + a.im * a.im);
+    return C64(a.re * s, (0.0 - a.im) * s);
+}
 // a += b * c (complex multiply-add)
 PWA_HD void cmac(C64& a, const C64& b, const C64& c) {
     a.re = fma(b.re, c.re, fma(-b.im, c.im, a.re));

@@ -182,8 +182,9 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             const double sig = S->mres[0] + mjp * (S->mres[1] + mjp * S->mres[2]);
             double acc = 0.0;
             C64 L[NZ];
-            if constexpr (GRAD && !FULL) {
-            // two sub-bins per pass (the second one zero-weighted past the end)
+            if constexpr (GRAD && !FULL && NZ <= 3) {
+            // two sub-bins per pass (the second one zero-weighted past the end);
+            // with more Pc resonances the doubled lineshape state exceeds 256 VGPRs
             for (int b = 0; b < nb; b += 2) {
                 const bool two = b + 1 < nb;
                 ChainKin kin[2];

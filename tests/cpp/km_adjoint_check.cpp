@@ -14,6 +14,7 @@ int main() {
     D.m0[0] = 4.31; D.m0[1] = 4.47;
     for (int q = 0; q < 8; ++q) D.km[0][q] = N(rng);
     for (int x = 0; x < 4; ++x) for (int y = x; y < 4; ++y) { double v = N(rng); D.km[0][16 + x * 4 + y] = v; D.km[0][16 + y * 4 + x] = v; }
+    km_pole_terms(S, D);
     C64 Ab[8]; for (int q = 0; q < 6; ++q) Ab[q] = C64(N(rng), N(rng));
     double worst = 0;
     for (double mpp : {4.2, 4.33, 4.46, 4.55}) {
@@ -23,7 +24,7 @@ int main() {
         for (int q = 0; q < 6; ++q) worst = fmax(worst, hypot(Mk[q].re - M0[q].re, Mk[q].im - M0[q].im));
         double gb[16] = {0}, fb[16] = {0}, mbp[4] = {0};
         kmatrix_adjoint(&D, 0, st, Ab, gb, fb, mbp);
-        auto F = [&](DevDyn& d2) { C64 M[8]; kmatrix(&S, &d2, 0, mpp, M); double f = 0; for (int q = 0; q < 6; ++q) f += re_mul(Ab[q], M[q]); return f; };
+        auto F = [&](DevDyn& d2) { km_pole_terms(S, d2); C64 M[8]; kmatrix(&S, &d2, 0, mpp, M); double f = 0; for (int q = 0; q < 6; ++q) f += re_mul(Ab[q], M[q]); return f; };
         auto fd = [&](double* x) { double h = 1e-6 * fmax(1.0, fabs(*x)); double x0 = *x; DevDyn d2 = D; 
             double* p = (double*)((char*)&d2 + ((char*)x - (char*)&D)); *p = x0 + h; double fp = F(d2); *p = x0 - h; double fm = F(d2); return (fp - fm) / (2 * h); };
         for (int q = 0; q < 8; ++q) { double a = gb[q], b = fd(&D.km[0][q]); printf("m=%.2f g[%d] an=% .8e fd=% .8e\n", mpp, q, a, b); worst = fmax(worst, fabs(a - b) / fmax(1e-3, fabs(b))); }
