@@ -309,12 +309,13 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
     }
 }
 
+// v[idx] for a uniform idx in 0..7 as a tree of selects (a loop of compares
+// is turned back into an indexed load, which moves v to scratch memory).
 __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
-    C64 r = v[0];
-#pragma unroll
-    for (int q = 1; q < 8; ++q)
-        if (q == idx) r = v[q];
-    return r;
+    const bool b0 = idx & 1, b1 = idx & 2, b2 = idx & 4;
+    const C64 a0 = b0 ? v[1] : v[0], a1 = b0 ? v[3] : v[2], a2 = b0 ? v[5] : v[4], a3 = b0 ? v[7] : v[6];
+    const C64 c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2;
+    return b2 ? c1 : c0;
 }
 
 // Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.

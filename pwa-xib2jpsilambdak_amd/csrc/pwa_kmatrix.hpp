@@ -21,6 +21,31 @@ PWA_HD C64 rho_cmass(double s, double m1, double m2, double hw) {
     return C64(r.re / s, r.im / s);
 }
 
+// Mtr[np + c] = v[c] (c = 0..3) and its counterpart v[c] = Abar[np + c] for
+// the uniform pole count np, as trees of selects on the bits of the offset:
+// a runtime array index would move the arrays to scratch memory.
+PWA_HD C64 km_sel4(const C64 v[4], int c) {
+    const C64 lo = (c & 1) ? v[1] : v[0], hi = (c & 1) ? v[3] : v[2];
+    return (c & 2) ? hi : lo;
+}
+PWA_HD C64 km_sel8(const C64 v[8], int q) {
+    const C64 a0 = (q & 1) ? v[1] : v[0], a1 = (q & 1) ? v[3] : v[2];
+    const C64 a2 = (q & 1) ? v[5] : v[4], a3 = (q & 1) ? v[7] : v[6];
+    const C64 b0 = (q & 2) ? a1 : a0, b1 = (q & 2) ? a3 : a2;
+    return (q & 4) ? b1 : b0;
+}
+PWA_HD void km_place_nr(int np, const C64 v[4], C64 Mtr[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int c = q - np;
+        if (c >= 0 && c < 4) Mtr[q] = km_sel4(v, c);
+    }
+}
+PWA_HD void km_take_nr(int np, const C64 Abar[8], C64 v[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = km_sel8(Abar, np + c);
+}
+
 // Production amplitudes of K-matrix class ik at mass mpp: Mtr[p] (poles) and
 // Mtr[np + c] (non-resonant channels), as Default_KMatrix
 // (src/cu_DPFPropogator.cu:588-813): T = row 0 of (D (1 - i K rho))^-1 with
@@ -125,14 +150,10 @@ static PWA_HD void kmatrix(const DevStatic* __restrict__ S, const DevDyn* __rest
         for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * ms[p], T[c]);
         Mtr[p] = acc * na;
     }
+    C64 v[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const C64 v = T[c] * (Dall * na);
-        // Mtr[np + c] = v, written with compile-time indices only
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == np + c) Mtr[q] = v;
-    }
+    for (int c = 0; c < 4; ++c) v[c] = T[c] * (Dall * na);
+    km_place_nr(np, v, Mtr);
 }
 
 // ------------------------------------------------------ K-matrix adjoint
@@ -268,13 +289,10 @@ PWA_HD void kmatrix_state(const DevStatic* __restrict__ S, const DevDyn* __restr
         for (int c = 0; c < 4; ++c) rmac(acc, g[p * 4 + c] * st.ms[p], st.T[c]);
         Mtr[p] = acc * st.na;
     }
+    C64 v[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const C64 v = st.T[c] * (Dall * st.na);
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == np + c) Mtr[q] = v;
-    }
+    for (int c = 0; c < 4; ++c) v[c] = st.T[c] * (Dall * st.na);
+    km_place_nr(np, v, Mtr);
 }
 
 // Reverse pass: Abar[q] is the adjoint of Mtr[q] (delta F = Re sum_q Abar_q dMtr_q).
@@ -286,12 +304,11 @@ PWA_HD void kmatrix_adjoint(const DevDyn* __restrict__ D, int ik, const KMState&
     const int np = st.np;
     C64 Tb[4];
     C64 Db(0.0, 0.0), msb[4];
+    C64 anr[4];
+    km_take_nr(np, Abar, anr);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        C64 an;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q == np + c) an = Abar[q];
+        const C64 an = anr[c];
         Tb[c] = an * (st.na * st.Dall);
         cmac(Db, an * st.na, st.T[c]);
     }

@@ -154,21 +154,33 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
         // the gradient is carried for at most one K-matrix class (the host
         // rejects models with more for pwa_eval_grad); the forward handles all
         const int ikc = (GRAD && S->km_nclass > 0) ? S->km_class[0] : -1;
+        // the quadratic-form coefficients p0, Md, Mo of this event; with many Pc
+        // members in the FULL gradient variant they are reloaded every sub-bin
+        // (from L2) instead of being held in registers across the loop
+        constexpr bool kReload = FULL && GRAD && NZ >= 4;
+        auto load_form = [&](const double* __restrict__ src) {
+#pragma unroll
+            for (int r = 0; r < (NZ > 0 ? NZ : 1); ++r) {
+                if (r >= NZ) break;
+                p0[r] = C64(src[(int64_t)(2 + 2 * r) * ld + i], src[(int64_t)(3 + 2 * r) * ld + i]);
+                Md[r] = src[(int64_t)(2 + 2 * NZ + r) * ld + i];
+#pragma unroll
+                for (int q = 0; q < r; ++q) {
+                    const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
+                    Mo[q][r] = C64(src[(int64_t)o * ld + i], src[(int64_t)(o + 1) * ld + i]);
+                }
+            }
+        };
         if constexpr (NZ > 0) {
+            if constexpr (!kReload) load_form(pcs);
 #pragma unroll
             for (int r = 0; r < NZ; ++r) {
-                p0[r] = C64(P(2 + 2 * r), P(3 + 2 * r));
-                Md[r] = P(2 + 2 * NZ + r);
                 gm[r] = 0.0;
                 gw[r] = 0.0;
                 BW1[r] = C64(0.0, 0.0);
                 B2d[r] = 0.0;
 #pragma unroll
-                for (int q = 0; q < r; ++q) {
-                    const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
-                    Mo[q][r] = C64(P(o), P(o + 1));
-                    B2o[q][r] = C64(0.0, 0.0);
-                }
+                for (int q = 0; q < r; ++q) B2o[q][r] = C64(0.0, 0.0);
             }
             if constexpr (GRAD) {
 #pragma unroll
@@ -235,6 +247,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                     pc_pfactors<NZ>(L, p0, Md, Mo, Pr);
                     acc += pc_value<NZ>(L, p0, Pr);
                 } else {
+                    if constexpr (kReload) load_form(opaque(pcs));
                     C64 Mt[NZ], dL0[NZ], dL1[NZ];
                     double Vx[NZ];
                     KMState st;
@@ -268,10 +281,14 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                         gm[r] += re_mul(Pr[r], dL0[r]);
                         gw[r] += re_mul(Pr[r], dL1[r]);
                         if (ikc >= 0 && S->ik[S->nR + r] == ikc) {
+                            // Ab[ikm] += Pr Vx, as selects: an indexed store would move Ab to scratch
                             const C64 add = Pr[r] * Vx[r];
+                            const int j = S->ikm[S->nR + r];
 #pragma unroll
-                            for (int q = 0; q < 8; ++q)
-                                if (q == S->ikm[S->nR + r]) Ab[q] = Ab[q] + add;
+                            for (int q = 0; q < 8; ++q) {
+                                const bool h = q == j;
+                                Ab[q] = Ab[q] + C64(h ? add.re : 0.0, h ? add.im : 0.0);
+                            }
                         }
                     }
                     if constexpr (FULL)
@@ -280,6 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             }
             f += acc * inb;
             if (MC && noint) {
+                if constexpr (kReload) load_form(pcs);
                 if (scheme != 0) pc_lineshapes<NZ, FULL>(S, D, mjp, L);
                 C64 zero[NZ];
                 fni += pc_form<NZ>(L, zero, Md, Mo);
