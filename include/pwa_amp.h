@@ -45,6 +45,17 @@ extern "C" {
 #define PWA_BREITWIGNER 1
 #define PWA_KMATRIX 240
 #define PWA_NONRESONANT 361
+/* Types with a defined parameter layout that this library does not evaluate:
+   pwa_create rejects a model that uses them with PWA_E_MODEL.  The spline
+   (MODELINDEPENDENT, Lambda* and Pc chains), revised K-matrix (REVISEDKMATRIX,
+   Lambda* and Pc) and triangle cusp (TRIANGLE, Pc) lineshapes of
+   src/kernel_calEva.cu:133-155, :404-452 are used by none of the shipped fit
+   configurations.                                                         */
+#define PWA_MODELINDEPENDENT 3
+#define PWA_TRIANGLE 7
+#define PWA_REVISEDKMATRIX 360
+#define PWA_NUMSP 24  /* spline knots, Lambda* chain (RooTDalitz/Resonances.h:5) */
+#define PWA_NUMSPZ 11 /* spline knots, Pc chain (RooTDalitz/Resonances.h:6)      */
 
 #define PWA_MAX_RES 64        /* resonances per model                              */
 #define PWA_MAX_SCHEMES 8     /* smearing schemes (scheme id 1..n_schemes)         */
@@ -148,11 +159,22 @@ typedef struct pwa_model_desc {
  *            (RooTDalitz/ResZKM_alternateCoupling.h:18-43)
  *   Zcs    : [ 4 x (Re,Im) LS, a_Bp, p_Bp, m0, width, type, 2J, P ]           15
  *            (RooTDalitz/ResX.h:1-28)
+ * A PWA_MODELINDEPENDENT (spline) resonance carries its knots as (Re, Im, x)
+ * triples in place of the usual middle entries (MakeSpline/MakeSplineZ,
+ * src/RooDalitzAmplitude.cxx:1721-1750):
+ *   Lambda*: [ 12 couplings, PWA_NUMSP x (Re, Im, x), scRe, scIm, m0, width,
+ *              type, 2J, P ]                                                  91
+ *            (RooTDalitz/ResL_fullPHSP_finnerBinning.h:1261-1279)
+ *   Pc     : [ a_Bp, p_Bp, 3 x (Re,Im) LS, PWA_NUMSPZ x (Re, Im, x), m0,
+ *              width, type, 2J, P ]                                           46
+ *            (RooTDalitz/ResZ.h:392-404)
  * The type/2J/P entries must agree with the model description.
  */
 #define PWA_NPAR_LAMBDA 19
 #define PWA_NPAR_PC 17
 #define PWA_NPAR_ZCS 15
+#define PWA_NPAR_LAMBDA_SPLINE (PWA_NPAR_LAMBDA + 3 * PWA_NUMSP)
+#define PWA_NPAR_PC_SPLINE (PWA_NPAR_PC - 4 + 3 * PWA_NUMSPZ)
 
 typedef struct pwa_ctx pwa_ctx;
 

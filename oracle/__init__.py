@@ -68,6 +68,10 @@ def lib(path=LIB_PATH):
             "orc_prepare_ex": (C.c_void_p, [C.c_int64, _dp, C.c_int64, _dp, _ip, C.c_int64, _dp, C.c_int64, _dp,
                                             _ip, C.c_int]),
             "orc_nll_prepared_ex": (C.c_int, [C.c_void_p, vp, _dp, C.c_int, C.c_int, _dp]),
+            "orc_lineshape": (C.c_int, [vp, _dp, C.c_int, C.c_double, _dp]),
+            "orc_spline_coeffs": (None, [C.c_int, _dp, _dp, _dp]),
+            "orc_cusp": (None, [vp, C.c_double, C.c_double, C.c_double, _dp]),
+            "orc_cusp_inner": (None, [vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -148,6 +152,34 @@ def kmatrix(desc, params, mpp, ik):
     out = np.zeros(16)
     lib().orc_kmatrix(dref(desc), dp(_d(params)), mpp, ik, dp(out))
     return out[0::2] + 1j * out[1::2]
+
+
+def lineshape(desc, params, r, m):
+    """Lineshape of resonance r at mass m of its chain, times its vertex factor."""
+    out = np.zeros(2)
+    if lib().orc_lineshape(dref(desc), dp(_d(params)), int(r), float(m), dp(out)):
+        raise RuntimeError("orc_lineshape failed")
+    return complex(out[0], out[1])
+
+
+def spline_coeffs(x, y):
+    """Not-a-knot cubic spline: (n, 4) array of (y, b, c, d) per knot."""
+    x, y = _d(x), _d(y)
+    out = np.zeros(4 * len(x))
+    lib().orc_spline_coeffs(len(x), dp(x), dp(y), dp(out))
+    return out.reshape(-1, 4)
+
+
+def cusp(desc, m_r, g_r, mpp):
+    out = np.zeros(2)
+    lib().orc_cusp(dref(desc), m_r, g_r, mpp, dp(out))
+    return complex(out[0], out[1])
+
+
+def cusp_inner(desc, alpha, m_r, g_r, mpp):
+    out = np.zeros(2)
+    lib().orc_cusp_inner(dref(desc), alpha, m_r, g_r, mpp, dp(out))
+    return complex(out[0], out[1])
 
 
 def filldlz(ev):

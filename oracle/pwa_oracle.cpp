@@ -19,7 +19,14 @@
 // single-channel single-pole Breit-Wigner limit; smearing with sigma->0 equal
 // to the unsmeared amplitude; NLL invariance under a global complex rescaling
 // of all couplings; phase-space angles against an independent numpy
-// restatement of tree.C.
+// restatement of tree.C.  The lineshape types no shipped fit uses
+// (tests/test_oracle_lineshapes_other.py): spline coefficients against scipy's
+// not-a-knot CubicSpline and exact reproduction of cubics; the revised
+// K-matrix against the pinned Breit-Wigner (one member, Lambda(1405)) and the
+// one-channel K-matrix structure; the cusp's closed form against quadrature
+// per Feynman parameter (equal, or off by the reference's branch rules only).
+// Unpinned: the spline's last-knot row, reached only at m exactly equal to
+// the upper cut, outside the physical region.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -167,14 +174,19 @@ int orbital_L(const Model& M, int r) {
     return LR;
 }
 
-int param_offset(const Model& M, int r) {
-    if (r < M.nR) return r * PWA_NPAR_LAMBDA;
-    if (r < M.nR + M.nZ) return M.nR * PWA_NPAR_LAMBDA + (r - M.nR) * PWA_NPAR_PC;
-    return M.nR * PWA_NPAR_LAMBDA + M.nZ * PWA_NPAR_PC + (r - M.nR - M.nZ) * PWA_NPAR_ZCS;
-}
+// RooArgList length per resonance: a spline (MODELINDEPENDENT) resonance holds
+// its knot triples as well (src/RooDalitzAmplitude.cxx:1721-1750).
 int param_size(const Model& M, int r) {
     const int ch = M.chain(r);
-    return ch == 0 ? PWA_NPAR_LAMBDA : (ch == 1 ? PWA_NPAR_PC : PWA_NPAR_ZCS);
+    const bool spline = M.d->type[r] == PWA_MODELINDEPENDENT;
+    if (ch == 0) return spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
+    if (ch == 1) return spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
+    return PWA_NPAR_ZCS;
+}
+int param_offset(const Model& M, int r) {
+    int off = 0;
+    for (int q = 0; q < r; ++q) off += param_size(M, q);
+    return off;
 }
 
 const int Hel1Two[6] = {1, 1, -1, -1, 3, -3}; // src/RooDalitzAmplitude.cxx:111
@@ -283,6 +295,88 @@ cd PartWaveCoffX(const double* ires, int size, int J, int lambdaZ, int lambdaPsi
 
 // Amplitude-level parameters built from the fit parameters: the content of
 // the reference's h_paraList (cu_init_data, src/RooDalitzAmplitude.cxx:3141-3210)
+// Cubic spline through (x_i, y_i), i < n (n >= 4, x increasing), with the
+// not-a-knot end conditions (third derivative continuous across x_1 and
+// x_{n-2}).  ROOT 6.24.6's TSpline3 (root_24.yml:8; not vendored), built
+// without end-condition options as src/RooDalitzAmplitude.cxx:1733-1734 and
+// :1747-1748 do, is de Boor's CUBSPL with that condition.  Writes the
+// per-knot polynomial (y, b, c, d) of TSpline3::GetCoeff, value
+// y + b dx + c dx^2 + d dx^3 on [x_i, x_{i+1}); the last knot's row is the
+// last interval's cubic re-expanded at x_{n-1}.  The slopes come from a
+// dense partial-pivot elimination of the n x n system.
+void spline_coeffs(int n, const double* x, const double* y, double* ybcd) {
+    std::vector<double> A(n * n, 0.0), rhs(n, 0.0), h(n - 1), dd(n - 1);
+    for (int i = 0; i + 1 < n; ++i) {
+        h[i] = x[i + 1] - x[i];
+        dd[i] = (y[i + 1] - y[i]) / h[i];
+    }
+    // d_i = (s_i + s_{i+1} - 2 dd_i) / h_i^2; not-a-knot: d_0 = d_1, d_{n-3} = d_{n-2}
+    auto nak = [&](int row, int a) {
+        const double ia = 1.0 / (h[a] * h[a]), ib = 1.0 / (h[a + 1] * h[a + 1]);
+        A[row * n + a] += ia;
+        A[row * n + a + 1] += ia - ib;
+        A[row * n + a + 2] -= ib;
+        rhs[row] = 2.0 * dd[a] * ia - 2.0 * dd[a + 1] * ib;
+    };
+    nak(0, 0);
+    for (int i = 1; i + 1 < n; ++i) {  // second-derivative continuity at x_i
+        A[i * n + i - 1] = h[i];
+        A[i * n + i] = 2.0 * (h[i - 1] + h[i]);
+        A[i * n + i + 1] = h[i - 1];
+        rhs[i] = 3.0 * (h[i] * dd[i - 1] + h[i - 1] * dd[i]);
+    }
+    nak(n - 1, n - 3);
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+        if (piv != c) {
+            for (int k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+            std::swap(rhs[c], rhs[piv]);
+        }
+        for (int r = c + 1; r < n; ++r) {
+            const double f = A[r * n + c] / A[c * n + c];
+            if (f == 0.0) continue;
+            for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+            rhs[r] -= f * rhs[c];
+        }
+    }
+    std::vector<double> s(n);
+    for (int r = n - 1; r >= 0; --r) {
+        double t = rhs[r];
+        for (int k = r + 1; k < n; ++k) t -= A[r * n + k] * s[k];
+        s[r] = t / A[r * n + r];
+    }
+    for (int i = 0; i + 1 < n; ++i) {
+        ybcd[4 * i] = y[i];
+        ybcd[4 * i + 1] = s[i];
+        ybcd[4 * i + 2] = (3.0 * dd[i] - 2.0 * s[i] - s[i + 1]) / h[i];
+        ybcd[4 * i + 3] = (s[i] + s[i + 1] - 2.0 * dd[i]) / (h[i] * h[i]);
+    }
+    const int l = n - 2;
+    ybcd[4 * (n - 1)] = y[n - 1];
+    ybcd[4 * (n - 1) + 1] = s[n - 1];
+    ybcd[4 * (n - 1) + 2] = ybcd[4 * l + 2] + 3.0 * ybcd[4 * l + 3] * h[l];
+    ybcd[4 * (n - 1) + 3] = ybcd[4 * l + 3];
+}
+
+// Spline table of one MODELINDEPENDENT resonance in the reference's
+// _paraSpline layout (src/RooDalitzAmplitude.cxx:2740-2763, :2824-2845):
+// x[n], then (y, b, c, d) x n of the real part, then of the imaginary part.
+// Knots are (Re, Im, x) triples from offset `noff` of the parameter list.
+std::vector<double> spline_table(const double* ires, int noff, int n) {
+    std::vector<double> x(n), re(n), im(n), t(9 * n);
+    for (int i = 0; i < n; ++i) {
+        re[i] = ires[noff + 3 * i];
+        im[i] = ires[noff + 3 * i + 1];
+        x[i] = ires[noff + 3 * i + 2];
+    }
+    for (int i = 0; i < n; ++i) t[i] = x[i];
+    spline_coeffs(n, x.data(), re.data(), &t[n]);
+    spline_coeffs(n, x.data(), im.data(), &t[5 * n]);
+    return t;
+}
+
 struct AmpParams {
     std::vector<double> m0, g0;          // per resonance
     std::vector<int> L;                  // per resonance
@@ -290,6 +384,7 @@ struct AmpParams {
     std::vector<int> ampcount;           // first vamp index per resonance
     double km[PWA_KM_NCLASS][2 * PWA_KM_NPOLE * PWA_KM_NCH]; // g[4][4] then f[4][4] per class
     std::vector<int> ikmcount;           // per resonance (Pc KMATRIX members), else -1
+    std::vector<std::vector<double>> spl; // per resonance: spline table (MODELINDEPENDENT), else empty
 };
 
 int km_class(int S2, int LR) { return (LR > 0) ? ((S2 == 3) ? 3 : 2) : ((S2 == 3) ? 1 : 0); }
@@ -304,6 +399,7 @@ void build_amp(const Model& M, const double* par, AmpParams& ap) {
     ap.vamp.clear();
     ap.ampcount.assign(n, 0);
     ap.ikmcount.assign(n, -1);
+    ap.spl.assign(n, std::vector<double>());
     std::memset(ap.km, 0, sizeof(ap.km));
     std::vector<double> galpha[4], fscat[4];
     std::vector<std::vector<double>> fextra[4]; // PWA_F_KM_FULL_F: f[i][i..3] of NR member i
@@ -330,6 +426,7 @@ void build_amp(const Model& M, const double* par, AmpParams& ap) {
                     ap.vamp.push_back(a);
                 }
             }
+            if (M.d->type[r] == PWA_MODELINDEPENDENT) ap.spl[r] = spline_table(ires, 12, PWA_NUMSP);
         } else if (ch == 1) {
             for (int k = 0; k <= 1; ++k) {
                 for (int helpsi = -1; helpsi <= 1; ++helpsi)
@@ -344,6 +441,7 @@ void build_amp(const Model& M, const double* par, AmpParams& ap) {
                         }
                     }
             }
+            if (M.d->type[r] == PWA_MODELINDEPENDENT) ap.spl[r] = spline_table(ires, 8, PWA_NUMSPZ);
             if (M.d->type[r] == PWA_KMATRIX) {
                 int ik = (S2 == 3) ? 1 : 0;
                 if (parity > 0) ik += 2;
@@ -596,6 +694,145 @@ cd BW_AMP2(double ma, double mb, double ffr, double m_r, double gamma_r, double 
     return factor * breit_wigner(m_r, gamma, m2pp);
 }
 
+// ------------------------------------------- lineshapes outside the fits
+// Types the reference's switch also handles but no shipped fit uses
+// (src/kernel_calEva.cu:133-155, :404-452, :708-763).
+
+// m_reff of the reference's barrier factors: the resonance mass above the
+// two-body threshold, a tanh map into the kinematic range below it, the
+// range midpoint for non-resonant terms (m_r < 0); src/cu_DPFPropogator.cu:875-885.
+double mreff_chain(double m_r, double m1_0, double m2_0, double m3_0, double m3, double m_lb) {
+    if (m_r > m1_0 + m2_0) return m_r;
+    if (m_r >= 0) {
+        const double mmin = m1_0 + m2_0, mmax = m_lb - m3_0;
+        return mmin + 0.5 * (mmax - mmin) * (1.0 + std::tanh((m_r - (mmin + mmax) / 2.) / (mmax - mmin)));
+    }
+    return (m1_0 + m2_0 + m_lb - m3) / 2.0;
+}
+
+// KMatrixTerm, src/cu_DPFPropogator.cu:1070-1115.  Always the K- p masses,
+// whichever chain calls it.  A non-resonant member (m_r < 0) contributes the
+// running width of the first such member met (fsc), undivided.
+double KMatrixTerm(const Phys& P, double m_r, double gamma_r, double mpp, int LR, double& fsc) {
+    const double m2pp = mpp * mpp;
+    const double pR = getp(mpp, P.m_p, P.m_k);
+    const double mreff = mreff_chain(m_r, P.m_p, P.m_k, P.m_jpsi, P.m_jpsi, P.m_lb);
+    const double pR0 = getp(mreff, P.m_p, P.m_k);
+    const double FR = F_BW(P.ffr, LR, pR, pR0);
+    const double gamma = gamma_r * std::pow(pR / pR0, 2 * LR + 1) * (m_r / mpp) * FR * FR;
+    if (m_r < 0) {
+        if (std::fabs(fsc) < 1e-10) fsc = gamma;
+        return fsc;
+    }
+    if (std::fabs(m_r * m_r - m2pp) < 1e-10) return 0;
+    return m_r * gamma / (m_r * m_r - m2pp);
+}
+
+// Revised_KMAMP, src/cu_DPFPropogator.cu:825-926: resonance nr of a one-channel
+// K-matrix built from every REVISEDKMATRIX member of its J^P class in
+// [irst, ired):  factor / (D - i totG) with D = m_r^2 - s (1 for m_r <= 0) and
+// totG = m_r Gamma_r(s) + D * sum_{j != r} KMatrixTerm_j.  Below the K- p
+// threshold the pole is the Lambda(1405) two-channel form (K- p and
+// Sigma pi, the latter's coupling 0.0505 fixed), which also shifts D by the
+// non-resonant term.
+cd Revised_KMAMP(const Model& M, const AmpParams& ap, const Phys& P, double mpp, int nr, int irst, int ired,
+                 bool pc_chain) {
+    const double m2pp = mpp * mpp;
+    double m1 = P.m_p, m2 = P.m_k, m3 = P.m_jpsi;
+    if (pc_chain) { m1 = P.m_jpsi; m2 = P.m_p; m3 = P.m_k; }
+    const double m_r = ap.m0[nr], gamma_r = ap.g0[nr];
+    const int J_r = M.d->spin2[nr], L_r = ap.L[nr], type_r = M.d->type[nr], P_r = M.d->parity[nr];
+    double totG = 0, fsc = 0, denorm = 1;
+    if (m_r > 0) denorm = m_r * m_r - m2pp;
+    for (int ir = irst; ir < ired; ++ir) {
+        if (M.d->type[ir] == type_r && M.d->spin2[ir] == J_r && M.d->parity[ir] == P_r) {
+            if (ir == nr) continue;
+            totG += KMatrixTerm(P, ap.m0[ir], ap.g0[ir], mpp, ap.L[ir], fsc) * denorm;
+        }
+    }
+    const double pR = getp(mpp, m1, m2);
+    double pR0 = getp(mreff_chain(m_r, m1, m2, m3, m3, P.m_lb), m1, m2);
+    double FR = F_BW(P.ffr, L_r, pR, pR0);
+    double factor = FR * std::pow(pR / pR0, L_r);
+    if (m_r > 0) {
+        double gamma = gamma_r;
+        if (m_r > m1 + m2) gamma = gamma_r * std::pow(pR / pR0, 2 * L_r + 1) * (m_r / mpp) * FR * FR;
+        if (m_r < m1 + m2) {
+            const double msig = 1.18937, mpi0 = 0.13957018;
+            const double pRSigPi = getp(mpp, msig, mpi0);
+            pR0 = getp(m_r, msig, mpi0);
+            const double sigpi = 0.0505 * std::pow(pRSigPi / pR0, 2 * L_r + 1) * (m_r / mpp);
+            gamma = gamma_r * std::pow(pR / pR0, 2 * L_r + 1) * (m_r / mpp) + sigpi;
+            denorm = denorm - fsc * sigpi * m_r;
+        }
+        totG += m_r * gamma;
+        factor = factor * bw_norm(m_r, gamma_r);
+    } else {
+        totG += gamma_r * std::pow(pR / pR0, 2 * L_r + 1) * (m_r / mpp) * FR * FR;
+    }
+    const double d2 = denorm * denorm + totG * totG;
+    return factor * cd(denorm / d2, totG / d2);
+}
+
+// ModelIndependentWave3 / ModelIndependentWave, src/cu_DPFPropogator.cu:1118-1221:
+// zero outside [xmin, xmax], else the cubic of the knot interval found by
+// bisection (the end cubics extrapolate between xmin/xmax and the outer knots).
+cd spline_wave(const double* t, int n, double xmin, double xmax, double x) {
+    if (x < xmin || x > xmax) return cd(0, 0);
+    int klow = 0, khig = n - 1;
+    if (x <= xmin) klow = 0;
+    else if (x >= xmax) klow = khig;
+    else {
+        while (khig - klow > 1) {
+            const int khalf = (klow + khig) / 2;
+            if (x > t[khalf]) klow = khalf;
+            else khig = khalf;
+        }
+    }
+    const double dx = x - t[klow];
+    const double* a = &t[n + 4 * klow];
+    const double* b = &t[5 * n + 4 * klow];
+    return cd(a[0] + dx * (a[1] + dx * (a[2] + dx * a[3])), b[0] + dx * (b[1] + dx * (b[2] + dx * b[3])));
+}
+
+// cuClogf / cuCatanf, RooTDalitz/cuComplex.h:658-679: log with the argument
+// from acos (principal branch), atan(z) = -i/2 log((1 + iz) / (1 - iz)).
+cd clog_ref(cd x) {
+    const double r = std::sqrt(x.real() * x.real() + x.imag() * x.imag());
+    double theta = std::acos(x.real() / r);
+    if (x.imag() < 0) theta = -theta;
+    return cd(std::log(r), theta);
+}
+cd catan_ref(cd x) {
+    const cd ii(0, 1);
+    return -0.5 * (ii * clog_ref((1.0 + ii * x) / (1.0 - ii * x)));
+}
+
+// GetCusp_AMP, src/cu_DPFPropogator.cu:1223-1343: triangle loop with a
+// chi_c1 exchange, 100-point midpoint rule in the Feynman parameter alpha of
+// the closed-form inner integral (1/r)[atan(b/r) - atan((b + 2(alpha-1))/r)],
+// r = sqrt(4c - b^2), divided by m^2.
+// get_b, get_c, this_func (src/cu_DPFPropogator.cu:1223-1300) at Feynman parameter alpha.
+cd cusp_inner(const Phys& P, double alpha, double m_r, double gamma_r, double mpp) {
+    const double m_chic1 = 3.51067;  // src/cu_DPFPropogator.cu:15
+    const cd ii(0, 1);
+    const double p12 = P.m_lb, p13 = P.m_k, m3 = P.m_p;
+    const cd m1(m_r, -gamma_r / 2.0);
+    const cd b = (1.0 / mpp / mpp) * (alpha * (p12 * p12 - p13 * p13 - mpp * mpp) + m3 * m3 - m_chic1 * m_chic1) +
+                 1.0 - 0.001 * ii;
+    const cd c = (1.0 / mpp / mpp) * (alpha * (m1 * m1 - m3 * m3) + m3 * m3 - alpha * (1.0 - alpha) * p13 * p13);
+    const cd r = csqrt_ref(4.0 * c - b * b);
+    return (1.0 / r) * (catan_ref(b / r) - catan_ref((b + 2.0 * (alpha - 1.0)) / r));
+}
+
+cd GetCusp_AMP(const Phys& P, double m_r, double gamma_r, double mpp) {
+    const int bins = 100;
+    const double dxa = 1.0 / bins;
+    cd sum(0, 0);
+    for (int i = 0; i < bins; ++i) sum += dxa * cusp_inner(P, i * dxa + dxa / 2, m_r, gamma_r, mpp);
+    return sum * (1.0 / mpp / mpp);
+}
+
 // Row 0 of S^-1 by the reference's transposed LU without pivoting
 // (solve_NxN<double2,double2,0,N>, RooTDalitz/cuSolve4x4.h:278-350).
 void solve_row0(const cd* S, int n, cd* x) {
@@ -844,19 +1081,28 @@ void filldlz(Dlz& z, const double* ev /* PWA_COL_* values */) {
 }
 
 // --------------------------------------------------------- amplitudes
-// GetAmpLz, src/kernel_calEva.cu:85-190 (BREITWIGNER and NONRESONANT types)
+// Lambda* lineshape for one resonance at mass mkp (the type switch of
+// src/kernel_calEva.cu:121-155), including BTerm.
+cd LambdaLineshape(const Model& M, const AmpParams& ap, const Phys& P, int ir, double mkp) {
+    double m0 = ap.m0[ir];
+    const double g0 = ap.g0[ir];
+    const int type = M.d->type[ir];
+    cd BW(0, 0);
+    if (type == PWA_BREITWIGNER) BW = BW_AMP(P, m0, g0, mkp, ap.L[ir]);
+    else if (type == PWA_REVISEDKMATRIX) BW = Revised_KMAMP(M, ap, P, mkp, ir, 0, M.nR, false);
+    else if (type == PWA_MODELINDEPENDENT) {
+        BW = spline_wave(ap.spl[ir].data(), PWA_NUMSP, P.m_p + P.m_k - 0.002, P.m_lb - P.m_jpsi + 0.002, mkp);
+        m0 = -0.5;
+    } else if (type == PWA_NONRESONANT) BW = cd(1.0, 0.0);
+    return BTerm(P, 0.5 * M.d->spin2[ir], mkp, m0) * BW;
+}
+
+// GetAmpLz, src/kernel_calEva.cu:85-190
 void GetAmpLz(const Model& M, const AmpParams& ap, const Phys& P, const Dlz& z, cd* H) {
     for (int ir = 0; ir < M.nR; ++ir) {
-        double m0 = ap.m0[ir];
-        const double g0 = ap.g0[ir];
         const int S2 = M.d->spin2[ir];
-        const int LR = ap.L[ir];
         const double S = 0.5 * S2;
-        const int type = M.d->type[ir];
-        cd BW(0, 0);
-        if (type == PWA_BREITWIGNER) BW = BW_AMP(P, m0, g0, z.mpk, LR);
-        else if (type == PWA_NONRESONANT) BW = cd(1.0, 0.0);
-        BW = BTerm(P, S, z.mpk, m0) * BW;
+        const cd BW = LambdaLineshape(M, ap, P, ir, z.mpk);
         int ampcount = ap.ampcount[ir];
         const int spin_index = (S2 - 1) / 2;
         for (int k = 0; k <= 1; ++k)
@@ -928,6 +1174,13 @@ cd PcLineshape(const Model& M, const AmpParams& ap, const Phys& P, int iPc, doub
         const int ik = km_class(S2, LR);
         if (std::abs(AKM[ik][0]) < 1e-10) KMatrix(M, ap, P, mpp, S2, (LR > 0) ? 1 : -1, ik, AKM[ik], clamp);
         BW = AKM[ik][ap.ikmcount[ir]];
+    } else if (type == PWA_MODELINDEPENDENT) {
+        BW = spline_wave(ap.spl[ir].data(), PWA_NUMSPZ, P.m_p + P.m_jpsi - 0.002, 4.6, mpp);
+        m0 = -0.5;
+    } else if (type == PWA_REVISEDKMATRIX) {
+        BW = Revised_KMAMP(M, ap, P, mpp, ir, M.nR, M.nR + M.nZ, true);
+    } else if (type == PWA_TRIANGLE) {
+        BW = GetCusp_AMP(P, m0, g0, mpp);
     } else if (type == PWA_NONRESONANT) {
         BW = cd(1.0, 0.0);
     }
@@ -1137,7 +1390,48 @@ void orc_solve_row0(const double* S, int n, double* x) {
 int orc_fit_param_count(const pwa_model_desc* d) {
     Model M;
     if (!model_from(d, M)) return -1;
-    return M.nR * PWA_NPAR_LAMBDA + M.nZ * PWA_NPAR_PC + M.nX * PWA_NPAR_ZCS;
+    return param_offset(M, M.n);
+}
+
+// Lineshape (times its Lambda_b vertex factor) of resonance r at mass m of
+// its chain: the per-resonance factor GetAmpLz / the Pc loops multiply into
+// the angular sums.  Zcs resonances use BW_AMPX.  out = (Re, Im).
+int orc_lineshape(const pwa_model_desc* d, const double* par, int r, double m, double* out) {
+    Model M;
+    if (!model_from(d, M) || r < 0 || r >= M.n) return -1;
+    AmpParams ap;
+    build_amp(M, par, ap);
+    Phys P = phys_from(d);
+    cd v;
+    const int ch = M.chain(r);
+    if (ch == 0) v = LambdaLineshape(M, ap, P, r, m);
+    else if (ch == 1) {
+        cd AKM[4][8];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 8; ++b) AKM[a][b] = 0;
+        v = PcLineshape(M, ap, P, r - M.nR, m, AKM, (d->flags & PWA_F_KM_CLAMP) != 0);
+    } else {
+        v = BTermX(P, 0.5 * d->spin2[r], m, ap.m0[r]) * BW_AMP2(P.m_k, P.m_jpsi, P.ffr, ap.m0[r], ap.g0[r], m, ap.L[r]);
+    }
+    out[0] = v.real();
+    out[1] = v.imag();
+    return 0;
+}
+
+// Not-a-knot cubic spline coefficients (y, b, c, d) per knot, 4n values.
+void orc_spline_coeffs(int n, const double* x, const double* y, double* ybcd) { spline_coeffs(n, x, y, ybcd); }
+
+// Triangle cusp amplitude without the vertex factor (GetCusp_AMP), and its
+// closed-form inner integral at one Feynman parameter.
+void orc_cusp(const pwa_model_desc* d, double m_r, double g_r, double mpp, double* out) {
+    const cd v = GetCusp_AMP(phys_from(d), m_r, g_r, mpp);
+    out[0] = v.real();
+    out[1] = v.imag();
+}
+void orc_cusp_inner(const pwa_model_desc* d, double alpha, double m_r, double g_r, double mpp, double* out) {
+    const cd v = cusp_inner(phys_from(d), alpha, m_r, g_r, mpp);
+    out[0] = v.real();
+    out[1] = v.imag();
 }
 
 // Amplitude-level parameters (interleaved complex vamp; km block 4 x 32).

@@ -40,9 +40,14 @@ static const int kHel1Two[6] = {1, 1, -1, -1, 3, -3};
 static const int kHel2[6] = {1, 0, 0, -1, 1, -1};
 
 static int param_offset(const pwa_model_desc& d, int r) {
-    if (r < d.n_lambda) return r * PWA_NPAR_LAMBDA;
-    if (r < d.n_lambda + d.n_pc) return d.n_lambda * PWA_NPAR_LAMBDA + (r - d.n_lambda) * PWA_NPAR_PC;
-    return d.n_lambda * PWA_NPAR_LAMBDA + d.n_pc * PWA_NPAR_PC + (r - d.n_lambda - d.n_pc) * PWA_NPAR_ZCS;
+    int off = 0;
+    for (int q = 0; q < r; ++q) {
+        const bool spline = d.type[q] == PWA_MODELINDEPENDENT;
+        if (q < d.n_lambda) off += spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
+        else if (q < d.n_lambda + d.n_pc) off += spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
+        else off += PWA_NPAR_ZCS;
+    }
+    return off;
 }
 
 bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {

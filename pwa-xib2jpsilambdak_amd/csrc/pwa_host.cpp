@@ -133,15 +133,18 @@ int orbital_L(const pwa_model_desc& d, int r) {
     return LR;
 }
 
-int param_offset(const pwa_model_desc& d, int r) {
-    if (r < d.n_lambda) return r * PWA_NPAR_LAMBDA;
-    if (r < d.n_lambda + d.n_pc) return d.n_lambda * PWA_NPAR_LAMBDA + (r - d.n_lambda) * PWA_NPAR_PC;
-    return d.n_lambda * PWA_NPAR_LAMBDA + d.n_pc * PWA_NPAR_PC + (r - d.n_lambda - d.n_pc) * PWA_NPAR_ZCS;
-}
+// RooArgList length of resonance r (a spline resonance also holds its knot
+// triples, src/RooDalitzAmplitude.cxx:1721-1750).
 int param_size(const pwa_model_desc& d, int r) {
-    if (r < d.n_lambda) return PWA_NPAR_LAMBDA;
-    if (r < d.n_lambda + d.n_pc) return PWA_NPAR_PC;
+    const bool spline = d.type[r] == PWA_MODELINDEPENDENT;
+    if (r < d.n_lambda) return spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
+    if (r < d.n_lambda + d.n_pc) return spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
     return PWA_NPAR_ZCS;
+}
+int param_offset(const pwa_model_desc& d, int r) {
+    int off = 0;
+    for (int q = 0; q < r; ++q) off += param_size(d, q);
+    return off;
 }
 
 int km_class_of(int S2, int LR) { return (LR > 0) ? ((S2 == 3) ? 3 : 2) : ((S2 == 3) ? 1 : 0); }
@@ -166,6 +169,9 @@ int validate(const pwa_model_desc& d, std::string& err) {
             members[km_class_of(d.spin2[r], orbital_L(d, r))]++;
         } else if (t == PWA_NONRESONANT) {
             if (zcs) { err = "non-resonant Zcs is not supported"; return PWA_E_MODEL; }
+        } else if (t == PWA_MODELINDEPENDENT || t == PWA_REVISEDKMATRIX || t == PWA_TRIANGLE) {
+            err = "lineshape type " + std::to_string(t) + " (spline / revised K-matrix / triangle) is not evaluated";
+            return PWA_E_MODEL;
         } else if (t != PWA_BREITWIGNER) {
             err = "unsupported lineshape type " + std::to_string(t);
             return PWA_E_MODEL;
@@ -655,7 +661,9 @@ extern "C" {
 
 int pwa_fit_param_count(const pwa_model_desc* m, int32_t* n) {
     if (!m || !n) return PWA_E_ARG;
-    *n = m->n_lambda * PWA_NPAR_LAMBDA + m->n_pc * PWA_NPAR_PC + m->n_zcs * PWA_NPAR_ZCS;
+    const int nr = m->n_lambda + m->n_pc + m->n_zcs;
+    if (m->n_lambda < 0 || m->n_pc < 0 || m->n_zcs < 0 || nr > PWA_MAX_RES) return PWA_E_MODEL;
+    *n = param_offset(*m, nr);
     return PWA_OK;
 }
 

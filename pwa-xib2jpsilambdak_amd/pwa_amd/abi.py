@@ -12,6 +12,10 @@ PWA_CHAIN_LAMBDA, PWA_CHAIN_PC, PWA_CHAIN_ZCS = 0, 1, 2
 PWA_BREITWIGNER = 1
 PWA_KMATRIX = 240
 PWA_NONRESONANT = 361
+PWA_MODELINDEPENDENT = 3
+PWA_TRIANGLE = 7
+PWA_REVISEDKMATRIX = 360
+PWA_NUMSP, PWA_NUMSPZ = 24, 11
 
 PWA_MAX_RES = 64
 PWA_MAX_SCHEMES = 8
@@ -28,6 +32,8 @@ PWA_F_KM_FULL_F = 0x8
 PWA_NCOL = 24
 PWA_DATA, PWA_MC = 0, 1
 PWA_NPAR_LAMBDA, PWA_NPAR_PC, PWA_NPAR_ZCS = 19, 17, 15
+PWA_NPAR_LAMBDA_SPLINE = PWA_NPAR_LAMBDA + 3 * PWA_NUMSP
+PWA_NPAR_PC_SPLINE = PWA_NPAR_PC - 4 + 3 * PWA_NUMSPZ
 
 COLUMNS = [
     "mLk", "mjpsiL", "mjpsik",
@@ -75,17 +81,14 @@ class ModelDesc(C.Structure):
         return PWA_CHAIN_ZCS
 
     def param_offset(self, r):
-        nl, npc = self.n_lambda, self.n_pc
-        if r < nl:
-            return r * PWA_NPAR_LAMBDA
-        if r < nl + npc:
-            return nl * PWA_NPAR_LAMBDA + (r - nl) * PWA_NPAR_PC
-        return nl * PWA_NPAR_LAMBDA + npc * PWA_NPAR_PC + (r - nl - npc) * PWA_NPAR_ZCS
+        return sum(self.param_size(q) for q in range(r))
 
     def param_size(self, r):
-        return (PWA_NPAR_LAMBDA, PWA_NPAR_PC, PWA_NPAR_ZCS)[self.chain(r)]
+        ch = self.chain(r)
+        if self.type[r] == PWA_MODELINDEPENDENT and ch != PWA_CHAIN_ZCS:
+            return PWA_NPAR_LAMBDA_SPLINE if ch == PWA_CHAIN_LAMBDA else PWA_NPAR_PC_SPLINE
+        return (PWA_NPAR_LAMBDA, PWA_NPAR_PC, PWA_NPAR_ZCS)[ch]
 
     @property
     def n_params(self):
-        return (self.n_lambda * PWA_NPAR_LAMBDA + self.n_pc * PWA_NPAR_PC
-                + self.n_zcs * PWA_NPAR_ZCS)
+        return self.param_offset(self.n_res)

@@ -176,6 +176,37 @@ def pc_res(rng, spin2, parity, m0, width, type_=abi.PWA_BREITWIGNER, g=None, fsc
     return Resonance(abi.PWA_CHAIN_PC, type_, spin2, parity, p, fl)
 
 
+def lambda_spline_res(rng, spin2, parity, knots_x, knots, sc=(1.0, 0.0)):
+    """Lambda* with the spline lineshape (type MODELINDEPENDENT): knots as
+    (Re, Im, x) triples after the 12 couplings (src/RooDalitzAmplitude.cxx:1721-1736);
+    knot values float, positions are constants."""
+    n = abi.PWA_NUMSP
+    if len(knots_x) != n or len(knots) != n:
+        raise ValueError(f"a Lambda* spline has {n} knots")
+    base = lambda_res(rng, spin2, parity, 0.95, 0.3, abi.PWA_MODELINDEPENDENT, sc)
+    p = base.params[:12] + [0.0] * (3 * n) + base.params[12:]
+    fl = base.floating[:12] + [False] * (3 * n) + [False] * 7
+    for i in range(n):
+        p[12 + 3 * i], p[13 + 3 * i], p[14 + 3 * i] = knots[i].real, knots[i].imag, knots_x[i]
+        fl[12 + 3 * i] = fl[13 + 3 * i] = True
+    return Resonance(abi.PWA_CHAIN_LAMBDA, abi.PWA_MODELINDEPENDENT, spin2, parity, p, fl)
+
+
+def pc_spline_res(rng, spin2, parity, knots_x, knots):
+    """Pc with the spline lineshape: knots as (Re, Im, x) triples after the
+    8 couplings, then m0, width, type, 2J, P (RooTDalitz/ResZ.h:392-404)."""
+    n = abi.PWA_NUMSPZ
+    if len(knots_x) != n or len(knots) != n:
+        raise ValueError(f"a Pc spline has {n} knots")
+    base = pc_res(rng, spin2, parity, -0.5, 0.3, abi.PWA_MODELINDEPENDENT)
+    p = base.params[:8] + [0.0] * (3 * n) + base.params[12:]
+    fl = base.floating[:8] + [False] * (3 * n) + [False] * 5
+    for i in range(n):
+        p[8 + 3 * i], p[9 + 3 * i], p[10 + 3 * i] = knots[i].real, knots[i].imag, knots_x[i]
+        fl[8 + 3 * i] = fl[9 + 3 * i] = True
+    return Resonance(abi.PWA_CHAIN_PC, abi.PWA_MODELINDEPENDENT, spin2, parity, p, fl)
+
+
 def zcs_res(rng, spin2, parity, m0, width):
     p = [0.0] * abi.PWA_NPAR_ZCS
     fl = [False] * abi.PWA_NPAR_ZCS
