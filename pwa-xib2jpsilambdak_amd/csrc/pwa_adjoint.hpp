@@ -23,6 +23,15 @@ constexpr int kLd = 65;    // row stride (odd: the transposed reads hit distinct
 // LDS bytes of a gradient kernel block (4 waves): stage, row indices, accumulators.
 __host__ __device__ constexpr int grad_lds_doubles(int NGp) { return 4 * kRows * kLd + 4 * NGp + 4 * kRows / 2; }
 
+// A pointer the compiler must treat as unknown where it is taken: loads
+// through it are neither hoisted nor forwarded from earlier stores, so values
+// behind it are not held in registers across the code before.
+template <class T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 struct Emitter {
     lds_f64* stage;  // [kRows][kLd]
     lds_i32* sidx;   // [kRows] gradient index of each staged row
@@ -84,12 +93,81 @@ struct Emitter {
 struct GradLayout {
     int ncpl;  // complex couplings: [0, 2 ncpl)
     int nres;  // (m0, width) pairs: [2 ncpl, 2 ncpl + 2 nres)
-    int ng;    // total: 2 ncpl + 2 nres + 36 * PWA_KM_NCLASS
+    int ng;    // total: 2 ncpl + 2 nres + 36 * PWA_KM_NCLASS + 2 PWA_NUMSP nspl
     __device__ __forceinline__ int cpl(int j) const { return 2 * j; }
     __device__ __forceinline__ int res(int r) const { return 2 * ncpl + 2 * r; }
     // per K-matrix class: g[16], f[16], pole masses [4]
     __device__ __forceinline__ int km(int ik) const { return 2 * ncpl + 2 * nres + 36 * ik; }
+    // per spline table: (Re, Im) of each knot value
+    __device__ __forceinline__ int spl(int s) const { return km(PWA_KM_NCLASS) + 2 * PWA_NUMSP * s; }
 };
+
+// ------------------------------------------------------- spline lineshape
+// Knot interval of m by bisection (TSpline3 / ModelIndependentWave3).  It
+// stops at n - 2: the reference's last-knot row (m exactly at the upper cut)
+// is the last interval's cubic, the same polynomial.
+__device__ __forceinline__ int spl_interval(const SplTab& T, double m) {
+    int lo = 0, hi = PWA_NUMSP - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (m > T.x[mid]) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Spline resonance r at m: value B p(m) and, in the derivative slots, the
+// per-event data of the knot adjoint: (B, t) and (k, h) with B the vertex
+// term at m0 = -0.5, k the interval, h its width and t = (m - x_k) / h.
+// Zero value and B outside [xmin, xmax].
+__device__ __forceinline__ void spline_generic(const DevStatic* __restrict__ S, const SplTab& T, int r, double m,
+                                               C64& v, C64& d0, C64& d1) {
+    if (!(m >= T.xmin && m <= T.xmax)) {
+        v = C64(0.0, 0.0);
+        d0 = C64(0.0, 0.0);
+        d1 = C64(0.0, 1.0);
+        return;
+    }
+    const int k = spl_interval(T, m);
+    const double h = T.x[k + 1] - T.x[k];
+    const double t = (m - T.x[k]) / h;
+    const double u = 1.0 - t;
+    const double h00 = (1.0 + 2.0 * t) * u * u, h01 = t * t * (3.0 - 2.0 * t);
+    const double h10 = h * t * u * u, h11 = -h * t * t * u;
+    const double pr = T.yr[k] * h00 + T.yr[k + 1] * h01 + T.sr[k] * h10 + T.sr[k + 1] * h11;
+    const double pi = T.yi[k] * h00 + T.yi[k + 1] * h01 + T.si[k] * h10 + T.si[k + 1] * h11;
+    const double B = vertex_lambda<double>(S->P, S->spin2[r], m, -0.5);
+    v = C64(B * pr, B * pi);
+    d0 = C64(B, t);
+    d1 = C64((double)k, h);
+}
+
+// Knot rows of a spline resonance: dF/dRe y_j = a2 Re(B c_j R), dF/dIm y_j =
+// -a2 Im(B c_j R), c_j(m) = d p(m) / d y_j the cardinal spline, from the
+// Hermite form with the slope matrix.
+__device__ __forceinline__ void spline_adjoint(const SplTab& T, C64 bt, C64 kh, double a2, C64 R, int base,
+                                               Emitter& em) {
+    const int k = (int)kh.re;
+    const double h = kh.im, t = bt.im, u = 1.0 - t;
+    const double h00 = (1.0 + 2.0 * t) * u * u, h01 = t * t * (3.0 - 2.0 * t);
+    const double h10 = h * t * u * u, h11 = -h * t * t * u;
+    const C64 w = (a2 * bt.re) * R;
+    const double* s0 = T.sm + k * PWA_NUMSP;
+    const double* s1 = s0 + PWA_NUMSP;
+    for (int c = 0; c < PWA_NUMSP; c += 8) {
+        em.begin(16);
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int j = c + jj;
+            double cj = s0[j] * h10 + s1[j] * h11;
+            cj += (j == k) ? h00 : 0.0;
+            cj += (j == k + 1) ? h01 : 0.0;
+            em.put(2 * jj, cj * w.re);
+            em.put(2 * jj + 1, -cj * w.im);
+        }
+        em.commit(16, base + 2 * c);
+    }
+}
 
 __device__ __forceinline__ C64 cval(const Cx<D2>& x) { return C64(x.re.v, x.im.v); }
 __device__ __forceinline__ C64 cder(const Cx<D2>& x, int k) { return C64(x.re.d[k], x.im.d[k]); }
@@ -116,6 +194,10 @@ __device__ __forceinline__ void chain_ls_grad(const DevStatic* __restrict__ S, c
 // (bw_lambda x vertex_lambda, bw_twobody x vertex_zcs).
 __device__ __forceinline__ void chain_ls_generic(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
                                                  double m, C64& v, C64& d0, C64& d1) {
+    if (r < S->nR && S->spl_of[r] >= 0) {
+        spline_generic(S, D->spl[S->spl_of[r]], r, m, v, d0, d1);
+        return;
+    }
     const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
     Cx<D2> Ls(D2(1.0), D2(0.0));
     if (r < S->nR) {
@@ -172,7 +254,7 @@ __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, cons
 // ---------------------------------------------------------- Lambda* adjoint
 __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                const C64* __restrict__ cpl, const double* __restrict__ ev,
-                                               const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
+                                               double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
                                                const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
     const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
     const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
@@ -236,11 +318,32 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                 }
                 em.commit(2 * nm, base + 2 * k * nm);
             }
-            em.begin(2);
-            em.put(0, a2 * re_mul(dL0, R));
-            em.put(1, a2 * re_mul(dL1, R));
-            em.commit(2, G.res(r));
+            if (S->spl_of[r] < 0) {
+                em.begin(2);
+                em.put(0, a2 * re_mul(dL0, R));
+                em.put(1, a2 * re_mul(dL1, R));
+                em.commit(2, G.res(r));
+            } else if (a2 != 0.0) {
+                // spline: R replaces the consumed value in the generic slot;
+                // spline_rows emits the knot rows after this loop
+                double* g = gen + (int64_t)(kGenPlanes * D->ls[r].slot) * ld + i;
+                g[0] = R.re;
+                g[ld] = R.im;
+            }
         }
+    }
+}
+
+// Knot rows of the spline resonances (after lambda_adjoint has left each
+// one's adjoint R in its generic slot; where a2 = 0 the slot still holds the
+// finite value and the rows are zero).
+__device__ __forceinline__ void spline_rows(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                            const double* __restrict__ gen, int64_t ld, int64_t i, double a2,
+                                            const GradLayout& G, Emitter& em) {
+    for (int s = 0; s < S->nspl; ++s) {
+        const double* g = gen + (int64_t)(kGenPlanes * D->ls[S->spl_res[s]].slot) * ld + i;
+        spline_adjoint(D->spl[s], C64(g[2 * ld], g[3 * ld]), C64(g[4 * ld], g[5 * ld]), a2, C64(g[0], g[ld]),
+                       G.spl(s), em);
     }
 }
 

@@ -78,7 +78,8 @@ bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {
                             h.A.push_back({off + 2 * ib, a});
                             ++ib;
                         }
-                    h.b = off + 12;
+                    // overall scale (scRe, scIm) at size - 7, after any spline knots
+                    h.b = off + 12 + (d.type[r] == PWA_MODELINDEPENDENT ? 3 * PWA_NUMSP : 0);
                     h.b_scale = true;
                     c.push_back(h);
                 }

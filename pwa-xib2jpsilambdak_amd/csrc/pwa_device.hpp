@@ -42,6 +42,20 @@ enum Plane {
     E_NPLANE
 };
 
+// Lambda*-chain spline lineshapes (type MODELINDEPENDENT; ModelIndependentWave3,
+// src/cu_DPFPropogator.cu:1118-1168), per evaluation: knot positions, complex
+// knot values y and not-a-knot slopes s, and the slope matrix sm (s = sm y, a
+// function of the positions only) that gives the knot gradient.  The cubic on
+// [x_k, x_k+1] is the Hermite form of (y_k, y_k+1, s_k, s_k+1).
+constexpr int kMaxSpline = 2;
+struct SplTab {
+    double x[PWA_NUMSP];
+    double yr[PWA_NUMSP], yi[PWA_NUMSP];
+    double sr[PWA_NUMSP], si[PWA_NUMSP];
+    double sm[PWA_NUMSP * PWA_NUMSP];  // row k: d s_k / d y_j
+    double xmin, xmax;                 // zero outside [xmin, xmax]
+};
+
 // Static model tables, uploaded once per context.
 struct DevStatic {
     Masses P;
@@ -53,6 +67,9 @@ struct DevStatic {
     int32_t ampcount[PWA_MAX_RES];  // first complex coupling of resonance r
     int32_t ik[PWA_MAX_RES];        // K-matrix J^P class of a Pc member, else -1
     int32_t ikm[PWA_MAX_RES];       // index into the class's production vector
+    int32_t spl_of[PWA_MAX_RES];    // spline table of a Lambda* spline resonance, else -1
+    int32_t nspl;
+    int32_t spl_res[kMaxSpline];    // resonance of each spline table
     // Lambda* resonances grouped by 2J (1,3,5,7,9): lam_idx[lam_start[g] .. lam_start[g+1])
     int32_t lam_start[6];
     int32_t lam_idx[PWA_MAX_RES];
@@ -85,6 +102,7 @@ struct DevDyn {
     double km_m2s[PWA_KM_NCLASS][PWA_KM_NPOLE];
     double km_dm2m[PWA_KM_NCLASS][PWA_KM_NPOLE];
     double km_dm2g[PWA_KM_NCLASS][PWA_KM_NPOLE];
+    SplTab spl[kMaxSpline];
 };
 
 // Per-block reduction record: compensated sums (hi, lo).

@@ -158,6 +158,7 @@ int validate(const pwa_model_desc& d, std::string& err) {
     for (int s = 0; s < d.n_schemes; ++s)
         if (d.scheme_nbins[s] < 1 || d.scheme_nbins[s] > PWA_MAX_SUBBINS) { err = "scheme_nbins out of range"; return PWA_E_MODEL; }
     int members[4] = {0, 0, 0, 0};
+    int nspl = 0;
     for (int r = 0; r < n; ++r) {
         const int t = d.type[r];
         if (d.parity[r] != 1 && d.parity[r] != -1) { err = "parity must be +-1"; return PWA_E_MODEL; }
@@ -169,8 +170,11 @@ int validate(const pwa_model_desc& d, std::string& err) {
             members[km_class_of(d.spin2[r], orbital_L(d, r))]++;
         } else if (t == PWA_NONRESONANT) {
             if (zcs) { err = "non-resonant Zcs is not supported"; return PWA_E_MODEL; }
-        } else if (t == PWA_MODELINDEPENDENT || t == PWA_REVISEDKMATRIX || t == PWA_TRIANGLE) {
-            err = "lineshape type " + std::to_string(t) + " (spline / revised K-matrix / triangle) is not evaluated";
+        } else if (t == PWA_MODELINDEPENDENT) {
+            if (r >= d.n_lambda) { err = "the spline lineshape is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
+            if (++nspl > kMaxSpline) { err = "at most 2 spline resonances are supported"; return PWA_E_MODEL; }
+        } else if (t == PWA_REVISEDKMATRIX || t == PWA_TRIANGLE) {
+            err = "lineshape type " + std::to_string(t) + " (revised K-matrix / triangle) is not evaluated";
             return PWA_E_MODEL;
         } else if (t != PWA_BREITWIGNER) {
             err = "unsupported lineshape type " + std::to_string(t);
@@ -199,6 +203,11 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
         S.ampcount[r] = cm.ampcount[r];
         S.ik[r] = -1;
         S.ikm[r] = -1;
+        S.spl_of[r] = -1;
+        if (d.type[r] == PWA_MODELINDEPENDENT && r < S.nR && S.nspl < kMaxSpline) {
+            S.spl_res[S.nspl] = r;
+            S.spl_of[r] = S.nspl++;
+        }
         if (d.type[r] == PWA_KMATRIX) {
             const int ik = km_class_of(d.spin2[r], S.L[r]);
             S.ik[r] = ik;
@@ -237,6 +246,89 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
     }
 }
 
+// Slope matrix of the not-a-knot cubic spline through knots x[0..n-1]
+// (n >= 4): the knot slopes are s = sm y.  The conditions are those of
+// de Boor's CUBSPL without end conditions, which ROOT's TSpline3 implements
+// (src/RooDalitzAmplitude.cxx:1721-1750 builds it without options): second
+// derivative continuous at the inner knots, third derivative continuous
+// across x_1 and x_{n-2}.  A s = Bm y is solved for all unit y at once by
+// elimination with partial pivoting.  False if the knots do not increase.
+bool spline_slope_matrix(int n, const double* x, double* sm) {
+    std::vector<double> h(n - 1), A((size_t)n * n, 0.0), Bm((size_t)n * n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) {
+        h[i] = x[i + 1] - x[i];
+        if (!(h[i] > 0.0)) return false;
+    }
+    // not-a-knot rows: (s_a + s_a+1 - 2 dd_a) / h_a^2 = (s_a+1 + s_a+2 - 2 dd_a+1) / h_a+1^2
+    auto nak = [&](int row, int a) {
+        const double p = 1.0 / (h[a] * h[a]), q = 1.0 / (h[a + 1] * h[a + 1]);
+        A[row * n + a] += p;
+        A[row * n + a + 1] += p - q;
+        A[row * n + a + 2] -= q;
+        Bm[row * n + a] += -2.0 * p / h[a];
+        Bm[row * n + a + 1] += 2.0 * p / h[a] + 2.0 * q / h[a + 1];
+        Bm[row * n + a + 2] += -2.0 * q / h[a + 1];
+    };
+    nak(0, 0);
+    for (int i = 1; i + 1 < n; ++i) {
+        A[i * n + i - 1] = h[i];
+        A[i * n + i] = 2.0 * (h[i - 1] + h[i]);
+        A[i * n + i + 1] = h[i - 1];
+        Bm[i * n + i - 1] = -3.0 * h[i] / h[i - 1];
+        Bm[i * n + i] = 3.0 * h[i] / h[i - 1] - 3.0 * h[i - 1] / h[i];
+        Bm[i * n + i + 1] = 3.0 * h[i - 1] / h[i];
+    }
+    nak(n - 1, n - 3);
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+        if (A[piv * n + c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[c * n + k], A[piv * n + k]);
+                std::swap(Bm[c * n + k], Bm[piv * n + k]);
+            }
+        for (int r = c + 1; r < n; ++r) {
+            const double f = A[r * n + c] / A[c * n + c];
+            if (f == 0.0) continue;
+            for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+            for (int k = 0; k < n; ++k) Bm[r * n + k] -= f * Bm[c * n + k];
+        }
+    }
+    for (int r = n - 1; r >= 0; --r)
+        for (int j = 0; j < n; ++j) {
+            double t = Bm[r * n + j];
+            for (int k = r + 1; k < n; ++k) t -= A[r * n + k] * sm[k * n + j];
+            sm[r * n + j] = t / A[r * n + r];
+        }
+    return true;
+}
+
+// Spline table of Lambda* spline resonance r from its knot triples (Re, Im, x)
+// at offset 12 of its parameter list (MakeSpline, src/RooDalitzAmplitude.cxx:1721-1736).
+bool fill_spline(const pwa_model_desc& d, const double* ires, SplTab& T) {
+    const int n = PWA_NUMSP;
+    for (int i = 0; i < n; ++i) {
+        T.yr[i] = ires[12 + 3 * i];
+        T.yi[i] = ires[12 + 3 * i + 1];
+        T.x[i] = ires[12 + 3 * i + 2];
+    }
+    if (!spline_slope_matrix(n, T.x, T.sm)) return false;
+    for (int k = 0; k < n; ++k) {
+        double a = 0.0, b = 0.0;
+        for (int j = 0; j < n; ++j) {
+            a += T.sm[k * n + j] * T.yr[j];
+            b += T.sm[k * n + j] * T.yi[j];
+        }
+        T.sr[k] = a;
+        T.si[k] = b;
+    }
+    T.xmin = d.m_p + d.m_k - 0.002;
+    T.xmax = d.m_lb - d.m_jpsi + 0.002;
+    return true;
+}
+
 // Per-evaluation parameters: masses/widths, K-matrix blocks (the g_alpha /
 // f_scat packing of ConfigureCouplings and cu_init_data,
 // src/RooDalitzAmplitude.cxx:2831-2916, 3175-3190), pole lists.
@@ -252,6 +344,11 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         D.g0[r] = ires[size - 4];
         const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
         D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
+        if (c.hs.spl_of[r] >= 0) {  // spline: evaluated per event by k_lsgen
+            D.ls[r] = LsConst{};
+            D.ls[r].kind = LS_GENERIC;
+            if (!fill_spline(d, ires, D.spl[c.hs.spl_of[r]])) return -1;
+        }
         if (D.ls[r].kind == LS_GENERIC && chain != CH_PC) D.ls[r].slot = ngen++;
         if (d.type[r] != PWA_KMATRIX) continue;
         const int ik = c.hs.ik[r];
@@ -285,6 +382,10 @@ int upload_params(pwa_ctx* c, const double* params) {
     // the previous evaluation has synchronised the stream, so the pinned
     // staging buffers (and any generic-lineshape buffer) are free to reuse
     c->ngen = build_dyn(*c, params, *c->hdyn);
+    if (c->ngen < 0) {
+        c->err = "spline knot positions must increase";
+        return PWA_E_ARG;
+    }
     c->pc_generic = false;
     for (int r = c->desc.n_lambda; r < c->desc.n_lambda + c->desc.n_pc; ++r)
         if (c->hdyn->ls[r].kind == LS_GENERIC) c->pc_generic = true;
@@ -630,8 +731,17 @@ void map_gradient(const pwa_ctx* c, const double* params, const std::vector<doub
     const int o_res = 2 * c->ncpl;
     const int o_km = o_res + 2 * c->nres;
     int npole[4] = {0, 0, 0, 0}, nnr[4] = {0, 0, 0, 0};
+    const int o_spl = o_km + PWA_KM_NCLASS * kGradNClassParams;
     for (int r = 0; r < c->nres; ++r) {
         const int off = param_offset(d, r), size = param_size(d, r);
+        if (c->hs.spl_of[r] >= 0) {  // knot values (Re, Im); positions, m0 and width enter nowhere
+            const double* gs = &gk[o_spl + 2 * PWA_NUMSP * c->hs.spl_of[r]];
+            for (int j = 0; j < PWA_NUMSP; ++j) {
+                grad[off + 12 + 3 * j] += gs[2 * j];
+                grad[off + 13 + 3 * j] += gs[2 * j + 1];
+            }
+            continue;
+        }
         grad[off + size - 5] += gk[o_res + 2 * r];
         if (d.type[r] != PWA_KMATRIX) {
             grad[off + size - 4] += gk[o_res + 2 * r + 1];
@@ -693,7 +803,7 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipMalloc(&c->dS, sizeof(DevStatic)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dD, sizeof(DevDyn)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dC, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
-    c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams;
+    c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams + 2 * PWA_NUMSP * c->hs.nspl;
     if (ensure_groups(c, 2) != PWA_OK) {
         pwa_destroy(c);
         return PWA_E_NOMEM;

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_lam(const double* __restrict__ ev
 
 __global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                    const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
-                                                   const C64* __restrict__ cpl, const double* __restrict__ gen,
+                                                   const C64* __restrict__ cpl, double* __restrict__ gen,
                                                    const double* __restrict__ amp, const double* __restrict__ awp,
                                                    GradLayout G, int NGp, double* __restrict__ gpart, int rpb) {
     extern __shared__ double lds[];
@@ -89,6 +89,17 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict_
 #pragma unroll
             for (int h = 0; h < 8; ++h) Y[h] = C64(A(2 * h), A(2 * h + 1));
             zcs_adjoint(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MJPSIK), Y, a2, G, em);
+        }
+    }
+    // knot rows of the spline resonances: a second pass over the same events,
+    // after the adjoint above has left each event's R in the generic slots
+    if (S->nspl > 0) {
+        const double* gsp = opaque(gen);
+        for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
+            const int64_t gi = base + threadIdx.x;
+            const bool valid = gi < n;
+            const int64_t i = sd.begin + (valid ? gi : n - 1);
+            spline_rows(S, D, gsp, ld, i, valid ? 2.0 * awp[i] : 0.0, G, em);
         }
     }
     if (em.cnt > 0) em.flush();

@@ -17,14 +17,6 @@
 
 namespace pwa {
 
-// A pointer the compiler must treat as unknown on every loop iteration: the
-// coupling loads of the per-segment loops stay inside the loop instead of
-// being hoisted and held in registers across it.
-template <class T>
-__device__ __forceinline__ const T* opaque(const T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
-}
 
 // ---------------------------------------------------- intermediate planes
 // pcs (quadratic form):  [H02, HNI, p0 (2 NZ), Md (NZ), Mo (NZ(NZ-1))]  -> 2 + 2 NZ + NZ^2

@@ -343,6 +343,44 @@ def config_kmatrix(seed=3, nbins=40) -> Model:
     return m
 
 
+# Spline knots of the model-independent Lambda* S wave: m(K- p) in GeV and the
+# starting (Re, Im) knot values, the 24-row table Fit_fullPHSP/toyr_fine_bin.dat
+# that RooTDalitz/ResL_fullPHSP_finnerBinning.h:1217-1236 reads.
+SWAVE_KNOTS = [
+    (1.435, 35.3636, 0.443954), (1.44, 35.3636, 0.443954), (1.45, 19.2272, -3.14483),
+    (1.5, 10.3566, -3.52221), (1.525, 10.3566, -3.52221), (1.55, 8.30692, -4.0512),
+    (1.575, 8.30692, -4.0512), (1.6, 8.26426, -4.53111), (1.625, 9.39549, -4.23039),
+    (1.65, 9.39549, -4.23039), (1.675, 10.4336, -3.65046), (1.7, 11.0594, -2.06806),
+    (1.75, 10.4866, 1.09322), (1.8, 8.38358, 3.30778), (1.85, 5.804, 4.20414),
+    (1.9, 3.24442, 3.9901), (1.95, 0.957838, 2.51269), (2.0, 0.168795, -0.480514),
+    (2.05, 2.32157, -2.63491), (2.1, 4.44767, -1.94504), (2.2, 4.78858, 0.260116),
+    (2.3, 4.00946, 1.10984), (2.4, 3.32141, 1.38757), (2.5, 2.79058, 1.45425),
+]
+
+
+def config_spline(seed=5, nbins=40) -> Model:
+    """Lambda* S wave as two model-independent splines (1/2-, the two coupling
+    sets of L_S01 / L_S01_2 in RooTDalitz/ResL_fullPHSP_finnerBinning.h:1261-1318,
+    shared knot positions, separate knot values) beside 8 Breit-Wigner Lambda*,
+    3 smeared Pc Breit-Wigners and 1 Zcs.  Knot values: the reference's
+    starting table scaled by 0.05, the second spline's perturbed by 10 %."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    _nominal_smearing(d, nbins)
+    kx = [k[0] for k in SWAVE_KNOTS]
+    kv = np.array([complex(k[1], k[2]) for k in SWAVE_KNOTS]) * 0.05
+    res = [lambda_spline_res(rng, 1, -1, kx, kv),
+           lambda_spline_res(rng, 1, -1, kx, kv * (1.0 + 0.1 * rng.standard_normal(len(kv))))]
+    res += [lambda_res(rng, s2, p, rng.uniform(1.50, 2.45), rng.uniform(0.02, 0.30))
+            for s2, p in [(3, -1), (3, 1), (5, -1), (5, 1), (7, 1), (1, 1), (3, -1), (9, 1)]]
+    for m0, (s2, p) in NOMINAL_PC:
+        res.append(pc_res(rng, s2, p, m0, rng.uniform(0.005, 0.03)))
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    m = assemble(d, res, "spline: 2 spline + 8 BW Lambda* + 3 Pc + 1 Zcs")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
     """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
     m = np.asarray(mjpsip)

@@ -98,6 +98,12 @@ def test_grad_kmatrix():
     _check_fd(models.config_kmatrix(3), 400, 800, 43)
 
 
+def test_grad_lambda_spline():
+    """Knot-value gradient of the Lambda* splines (cardinal basis from the
+    slope matrix) and the rest of the spline model."""
+    _check_fd(models.config_spline(5), 400, 800, 44)
+
+
 def test_fcn_gradient_with_constraints():
     model = models.config_small(1)
     dcols, sw, ds, mcols, wmc, ms = _setup(model, 800, 1600, 44)

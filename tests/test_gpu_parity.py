@@ -93,3 +93,10 @@ def test_unsmeared_amp2_matches_oracle():
         c.upload(abi.PWA_MC, cols, np.ones(cols.shape[1]))
         a2 = c.amp2(abi.PWA_MC, model.params)
     assert np.max(np.abs(a2 - ref) / ref) < FX_RTOL
+
+
+def test_lambda_spline_smeared_pc():
+    """Two model-independent Lambda* S-wave splines (type MODELINDEPENDENT,
+    evaluated per event by k_lsgen) with smeared Pc Breit-Wigners."""
+    _check(models.config_spline(5), 1500, 3000, 25)
+
