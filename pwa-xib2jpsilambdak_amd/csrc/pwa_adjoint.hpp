@@ -198,6 +198,25 @@ __device__ __forceinline__ void chain_ls_generic(const DevStatic* __restrict__ S
         spline_generic(S, D->spl[S->spl_of[r]], r, m, v, d0, d1);
         return;
     }
+    if (r < S->nR && S->rkm_cls[r] >= 0) {  // value only; rkm_rows emits the gradient
+        const int c = S->rkm_cls[r], n = S->rkm_nmem[c];
+        double mv[kRkmMem], gv[kRkmMem];
+        uint32_t Lpk = 0;
+        int self = 0;
+#pragma unroll
+        for (int q = 0; q < kRkmMem; ++q) {
+            const int rq = S->rkm_mem[c][q < n ? q : 0];
+            mv[q] = D->m0[rq];
+            gv[q] = D->g0[rq];
+            Lpk |= (uint32_t)(S->L[rq] & 0xff) << (8 * q);
+            if (q < n && rq == r) self = q;
+        }
+        const C64 a = rkm_lambda<double>(S->P, mv, gv, Lpk, n, self, m);
+        v = a * vertex_lambda<double>(S->P, S->spin2[r], m, D->m0[r]);
+        d0 = C64(0.0, 0.0);
+        d1 = C64(0.0, 0.0);
+        return;
+    }
     const D2 m0 = D2::var(D->m0[r], 0), g0 = D2::var(D->g0[r], 1);
     Cx<D2> Ls(D2(1.0), D2(0.0));
     if (r < S->nR) {
@@ -318,18 +337,69 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                 }
                 em.commit(2 * nm, base + 2 * k * nm);
             }
-            if (S->spl_of[r] < 0) {
+            if (!S->defer[r]) {
                 em.begin(2);
                 em.put(0, a2 * re_mul(dL0, R));
                 em.put(1, a2 * re_mul(dL1, R));
                 em.commit(2, G.res(r));
             } else if (a2 != 0.0) {
-                // spline: R replaces the consumed value in the generic slot;
-                // spline_rows emits the knot rows after this loop
+                // spline / revised K-matrix: R replaces the consumed value in
+                // the generic slot; spline_rows / rkm_rows emit the rows after
                 double* g = gen + (int64_t)(kGenPlanes * D->ls[r].slot) * ld + i;
                 g[0] = R.re;
                 g[ld] = R.im;
             }
+        }
+    }
+}
+
+// (m0, width) rows of the revised K-matrix members: every member's lineshape
+// depends on every member's mass and width, so per class and per member j
+// the class's lineshapes x vertex terms are evaluated with (m_j, Gamma_j)
+// as the dual variables and contracted with the members' adjoints R (left in
+// their generic slots by lambda_adjoint).
+__device__ __forceinline__ void rkm_rows(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                         const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
+                                         double a2, const GradLayout& G, Emitter& em) {
+    for (int c = 0; c < S->rkm_ncls; ++c) {
+        const int n = S->rkm_nmem[c];
+        double mv[kRkmMem], gv[kRkmMem];
+        uint32_t Lpk = 0, Rpk = 0;  // orbital momenta, resonance indices: byte q of member q
+        C64 Rt[kRkmMem];
+#pragma unroll
+        for (int q = 0; q < kRkmMem; ++q) {
+            const int rq = S->rkm_mem[c][q < n ? q : 0];
+            Rpk |= (uint32_t)(rq & 0xff) << (8 * q);
+            mv[q] = D->m0[rq];
+            gv[q] = D->g0[rq];
+            Lpk |= (uint32_t)(S->L[rq] & 0xff) << (8 * q);
+            const double* g = gen + (int64_t)(kGenPlanes * D->ls[rq].slot) * ld + i;
+            Rt[q] = (q < n) ? C64(g[0], g[ld]) : C64(0.0, 0.0);
+        }
+        for (int j = 0; j < n; ++j) {
+            D2 m[kRkmMem], g[kRkmMem];
+#pragma unroll
+            for (int q = 0; q < kRkmMem; ++q) {  // seeds as values, not as an indexed store
+                const double on = (double)(q == j);
+                m[q] = D2(mv[q]);
+                m[q].d[0] = on;
+                g[q] = D2(gv[q]);
+                g[q].d[1] = on;
+            }
+            double d0 = 0.0, d1 = 0.0;
+#pragma unroll 1
+            for (int r = 0; r < n; ++r) {
+                const D2 mr = sel4(m, r);
+                const C64 R = sel4(Rt, r);
+                const int res = (int)((Rpk >> (8 * r)) & 0xffu);
+                const Cx<D2> Lr = rkm_lambda<D2>(S->P, m, g, Lpk, n, r, mkp) * vertex_lambda<D2>(S->P, S->spin2[res], mkp, mr);
+                d0 += re_mul(cder(Lr, 0), R);
+                d1 += re_mul(cder(Lr, 1), R);
+            }
+            em.begin(2);
+            em.put(0, a2 * d0);
+            em.put(1, a2 * d1);
+            em.commit(2, G.res((int)((Rpk >> (8 * j)) & 0xffu)));
         }
     }
 }

@@ -70,6 +70,14 @@ struct DevStatic {
     int32_t spl_of[PWA_MAX_RES];    // spline table of a Lambda* spline resonance, else -1
     int32_t nspl;
     int32_t spl_res[kMaxSpline];    // resonance of each spline table
+    // Lambda* revised K-matrix classes: members (resonance indices, list order)
+    int32_t rkm_cls[PWA_MAX_RES];   // class of a member, else -1
+    int32_t rkm_ncls;
+    int32_t rkm_nmem[kRkmCls];
+    int32_t rkm_mem[kRkmCls][kRkmMem];
+    // Lambda* resonances whose parameter gradient a second pass emits
+    // (splines, revised K-matrix members): 1, else 0
+    int32_t defer[PWA_MAX_RES];
     // Lambda* resonances grouped by 2J (1,3,5,7,9): lam_idx[lam_start[g] .. lam_start[g+1])
     int32_t lam_start[6];
     int32_t lam_idx[PWA_MAX_RES];
@@ -152,6 +160,7 @@ struct StageArgs {
     double* fx;           // per-event output (device order) or null
     Partial* part;        // nblocks partial sums
     int ncpl, nres, ng;   // gradient layout
+    int nrkm;             // revised K-matrix classes (k_adj_rkm runs after k_adj_lam)
     double* gpart;        // gradient partial rows: block b of k_shape or k_pcdirect / k_adj_lam / k_adj_pc
                           // -> rows rpb b + 0 / 1 / 2
     int rpb;              // gradient rows per block (2, or 3 with Pc-chain resonances)

@@ -173,8 +173,10 @@ int validate(const pwa_model_desc& d, std::string& err) {
         } else if (t == PWA_MODELINDEPENDENT) {
             if (r >= d.n_lambda) { err = "the spline lineshape is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
             if (++nspl > kMaxSpline) { err = "at most 2 spline resonances are supported"; return PWA_E_MODEL; }
-        } else if (t == PWA_REVISEDKMATRIX || t == PWA_TRIANGLE) {
-            err = "lineshape type " + std::to_string(t) + " (revised K-matrix / triangle) is not evaluated";
+        } else if (t == PWA_REVISEDKMATRIX) {
+            if (r >= d.n_lambda) { err = "the revised K-matrix is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
+        } else if (t == PWA_TRIANGLE) {
+            err = "the triangle-cusp lineshape is not evaluated";
             return PWA_E_MODEL;
         } else if (t != PWA_BREITWIGNER) {
             err = "unsupported lineshape type " + std::to_string(t);
@@ -183,6 +185,20 @@ int validate(const pwa_model_desc& d, std::string& err) {
     }
     for (int k = 0; k < 4; ++k)
         if (members[k] > 8) { err = "more than 8 members in a K-matrix class"; return PWA_E_MODEL; }
+    // revised K-matrix classes (2J, P) of the Lambda* chain
+    int key[kRkmCls + 1], cnt[kRkmCls + 1], ncls = 0;
+    for (int r = 0; r < d.n_lambda; ++r) {
+        if (d.type[r] != PWA_REVISEDKMATRIX) continue;
+        const int k = d.spin2[r] * 2 + (d.parity[r] > 0 ? 1 : 0);
+        int c = 0;
+        while (c < ncls && key[c] != k) ++c;
+        if (c == ncls) {
+            if (ncls == kRkmCls) { err = "at most 4 revised K-matrix classes are supported"; return PWA_E_MODEL; }
+            key[ncls] = k;
+            cnt[ncls++] = 0;
+        }
+        if (++cnt[c] > kRkmMem) { err = "at most 4 members in a revised K-matrix class"; return PWA_E_MODEL; }
+    }
     return PWA_OK;
 }
 
@@ -204,9 +220,21 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
         S.ik[r] = -1;
         S.ikm[r] = -1;
         S.spl_of[r] = -1;
+        S.rkm_cls[r] = -1;
         if (d.type[r] == PWA_MODELINDEPENDENT && r < S.nR && S.nspl < kMaxSpline) {
             S.spl_res[S.nspl] = r;
             S.spl_of[r] = S.nspl++;
+            S.defer[r] = 1;
+        }
+        if (d.type[r] == PWA_REVISEDKMATRIX && r < S.nR) {  // class = (2J, P), members in list order
+            int c = 0;
+            while (c < S.rkm_ncls && !(d.spin2[S.rkm_mem[c][0]] == d.spin2[r] && d.parity[S.rkm_mem[c][0]] == d.parity[r])) ++c;
+            if (c == S.rkm_ncls && S.rkm_ncls < kRkmCls) S.rkm_nmem[S.rkm_ncls++] = 0;
+            if (c < S.rkm_ncls && S.rkm_nmem[c] < kRkmMem) {
+                S.rkm_mem[c][S.rkm_nmem[c]++] = r;
+                S.rkm_cls[r] = c;
+                S.defer[r] = 1;
+            }
         }
         if (d.type[r] == PWA_KMATRIX) {
             const int ik = km_class_of(d.spin2[r], S.L[r]);
@@ -344,10 +372,10 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         D.g0[r] = ires[size - 4];
         const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
         D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
-        if (c.hs.spl_of[r] >= 0) {  // spline: evaluated per event by k_lsgen
+        if (c.hs.defer[r]) {  // spline / revised K-matrix: evaluated per event by k_lsgen
             D.ls[r] = LsConst{};
             D.ls[r].kind = LS_GENERIC;
-            if (!fill_spline(d, ires, D.spl[c.hs.spl_of[r]])) return -1;
+            if (c.hs.spl_of[r] >= 0 && !fill_spline(d, ires, D.spl[c.hs.spl_of[r]])) return -1;
         }
         if (D.ls[r].kind == LS_GENERIC && chain != CH_PC) D.ls[r].slot = ngen++;
         if (d.type[r] != PWA_KMATRIX) continue;
@@ -544,6 +572,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.ncpl = c->ncpl;
         a.nres = c->nres;
         a.ng = NG;
+        a.nrkm = c->hs.rkm_ncls;
         a.gpart = grad ? c->gpart + R0[k] * NG : nullptr;
         a.rpb = rpb;
         const int li = c->nlaunch < pwa_ctx::kMaxLaunch ? c->nlaunch++ : -1;

@@ -91,21 +91,46 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict_
             zcs_adjoint(S, D, cpl, ev, gen, ld, i, ld_d(ev, ld, i, E_MJPSIK), Y, a2, G, em);
         }
     }
-    // knot rows of the spline resonances: a second pass over the same events,
-    // after the adjoint above has left each event's R in the generic slots
+    // rows of the spline knots: a second pass over the same events, after the
+    // adjoint above has left each event's R in the generic slots
     if (S->nspl > 0) {
         const double* gsp = opaque(gen);
         for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
             const int64_t gi = base + threadIdx.x;
             const bool valid = gi < n;
             const int64_t i = sd.begin + (valid ? gi : n - 1);
-            spline_rows(S, D, gsp, ld, i, valid ? 2.0 * awp[i] : 0.0, G, em);
+            const double a2 = valid ? 2.0 * awp[i] : 0.0;
+            spline_rows(S, D, gsp, ld, i, a2, G, em);
         }
     }
     if (em.cnt > 0) em.flush();
     __syncthreads();
     for (int q = threadIdx.x; q < G.ng; q += kBlock)
         gpart[((int64_t)rpb * blockIdx.x + 1) * G.ng + q] = Emitter::block_sum(lds, NGp, q);
+}
+
+// Revised K-matrix (m0, width) rows, after k_adj_lam on the same blocks and
+// events: added into k_adj_lam's gradient row of each block.  A kernel of
+// its own so that its dual-number evaluation does not weigh on k_adj_lam's
+// registers.
+__global__ void __launch_bounds__(kBlock, 1) k_adj_rkm(const double* __restrict__ ev, int64_t ld, SegView sv,
+                                                   const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                   const double* __restrict__ gen, const double* __restrict__ awp,
+                                                   GradLayout G, int NGp, double* __restrict__ gpart, int rpb) {
+    extern __shared__ double lds[];
+    Emitter em = Emitter::make(lds, NGp);
+    const SegDesc sd = sv.seg();
+    const int64_t n = sd.n;
+    for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
+        const int64_t gi = base + threadIdx.x;
+        const bool valid = gi < n;
+        const int64_t i = sd.begin + (valid ? gi : n - 1);
+        rkm_rows(S, D, gen, ld, i, ld_d(ev, ld, i, E_MKP), valid ? 2.0 * awp[i] : 0.0, G, em);
+    }
+    if (em.cnt > 0) em.flush();
+    __syncthreads();
+    for (int q = threadIdx.x; q < G.ng; q += kBlock)
+        gpart[((int64_t)rpb * blockIdx.x + 1) * G.ng + q] += Emitter::block_sum(lds, NGp, q);
 }
 
 hipError_t launch_lam(const StageArgs& a, double* pcs, double* amp, bool store_h0, hipStream_t st) {
@@ -128,6 +153,12 @@ hipError_t launch_adj_lam(const StageArgs& a, const double* amp, const double* a
     const int NGp = (a.ng + 1) & ~1;
     hipLaunchKernelGGL(k_adj_lam, dim3((unsigned)a.nblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S, a.D,
                        a.cpl, a.gen, amp, awp, G, NGp, a.gpart, a.rpb);
+    if ((e = hipGetLastError()) != hipSuccess || a.nrkm == 0) return e;
+    if ((e = hipFuncSetAttribute((const void*)k_adj_rkm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) !=
+        hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_adj_rkm, dim3((unsigned)a.nblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S, a.D, a.gen,
+                       awp, G, NGp, a.gpart, a.rpb);
     return hipGetLastError();
 }
 

@@ -351,6 +351,91 @@ PWA_HD Cx<T> bw_lambda(const Masses& P, T m_r, T g_r, double mpp, int LR) {
     return bw_prop(m_r, gamma, mpp * mpp) * factor;
 }
 
+// ----------------------------------------- revised K-matrix (Lambda* chain)
+// A J^P class of REVISEDKMATRIX Lambda* resonances forms a one-channel
+// K-matrix K(s) = sum_q m_q Gamma_q(s) / (m_q^2 - s) (+ the running width of
+// its first non-resonant member); member r's lineshape is
+// factor_r / (D_r - i totG_r) with D_r = m_r^2 - s and totG_r = m_r Gamma_r(s)
+// + D_r sum_{q != r} K_q (Revised_KMAMP, src/cu_DPFPropogator.cu:825-926).
+constexpr int kRkmMem = 4;  // members per class
+constexpr int kRkmCls = 4;  // classes
+
+// a[i] of a 4-entry register array by a tree of selects (an indexed access
+// would put the array in scratch memory)
+template <class T>
+PWA_HD T sel4(const T (&a)[4], int i) {
+    const T lo = (i & 1) ? a[1] : a[0], hi = (i & 1) ? a[3] : a[2];
+    return (i & 2) ? hi : lo;
+}
+
+// Effective mass of the barrier normalisation: the mass above threshold, the
+// tanh map below it, the range midpoint for non-resonant members.
+template <class T>
+PWA_HD T rkm_mreff(const Masses& P, T m) {
+    const double thr = P.m_p + P.m_k, mmax = P.m_lb - P.m_jpsi;
+    const double mv = val(m);
+    if (mv > thr) return m;
+    if (mv >= 0) return tanh_mass(m, thr, mmax);
+    return T(0.5 * (thr + mmax));
+}
+
+// KMatrixTerm (src/cu_DPFPropogator.cu:1070-1115): m Gamma(s) / (m^2 - s), or
+// for a non-resonant member (m < 0) the running width of the first one met.
+template <class T>
+PWA_HD T rkm_term(const Masses& P, T mq, T gq, int Lq, double mpp, double pR, T& fsc) {
+    const T pR0 = breakup(rkm_mreff(P, mq), P.m_p, P.m_k);
+    const T FR = barrier_ratio(P.ffr, Lq, T(pR), pR0);
+    const T gamma = gq * ipow(pR / pR0, 2 * Lq + 1) * (mq / mpp) * FR * FR;
+    const double mqv = val(mq);
+    if (mqv < 0) {
+        if (fabs(val(fsc)) < 1e-10) fsc = gamma;
+        return fsc;
+    }
+    const double s = mpp * mpp;
+    if (fabs(mqv * mqv - s) < 1e-10) return T(0.0);
+    return mq * gamma / (mq * mq - s);
+}
+
+// Member r of a class with members q < n (list order) at m(K p) = mpp;
+// member q's orbital momentum is byte q of Lpk (a packed word, not an int
+// array: the compiler turns a select over an int array into an indexed load).
+template <class T>
+PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRkmMem], uint32_t Lpk, int n, int r,
+                        double mpp) {
+    const double s = mpp * mpp, thr = P.m_p + P.m_k;
+    const double pR = breakup_d(mpp, P.m_p, P.m_k);
+    const T mr = sel4(m, r), gr = sel4(g, r);
+    const int Lr = (int)((Lpk >> (8 * r)) & 0xffu);
+    const double mrv = val(mr);
+    T denorm = (mrv > 0) ? mr * mr - s : T(1.0);
+    T totG(0.0), fsc(0.0);
+#pragma unroll
+    for (int q = 0; q < kRkmMem; ++q) {
+        if (q >= n || q == r) continue;
+        totG = totG + rkm_term(P, m[q], g[q], (int)((Lpk >> (8 * q)) & 0xffu), mpp, pR, fsc) * denorm;
+    }
+    const T pR0 = breakup(rkm_mreff(P, mr), P.m_p, P.m_k);
+    const T FR = barrier_ratio(P.ffr, Lr, T(pR), pR0);
+    T factor = FR * ipow(pR / pR0, Lr);
+    if (mrv > 0) {
+        T gamma = gr;
+        if (mrv > thr) gamma = gr * ipow(pR / pR0, 2 * Lr + 1) * (mr / mpp) * FR * FR;
+        if (mrv < thr) {  // Lambda(1405): K- p and Sigma pi widths
+            const double msig = 1.18937, mpi0 = 0.13957018;
+            const T p0s = breakup(mr, msig, mpi0);
+            const T sig = 0.0505 * ipow(breakup_d(mpp, msig, mpi0) / p0s, 2 * Lr + 1) * (mr / mpp);
+            gamma = gr * ipow(pR / p0s, 2 * Lr + 1) * (mr / mpp) + sig;
+            denorm = denorm - fsc * sig * mr;
+        }
+        totG = totG + mr * gamma;
+        factor = factor * bw_normalisation(mr, gr);
+    } else {
+        totG = totG + gr * ipow(pR / pR0, 2 * Lr + 1) * (mr / mpp) * FR * FR;
+    }
+    const T inv = 1.0 / (denorm * denorm + totG * totG);
+    return Cx<T>(factor * denorm * inv, factor * totG * inv);
+}
+
 // Breit-Wigner of a two-body system (ma, mb): the Pc (J/psi p) and Zcs (J/psi K) form.
 template <class T>
 PWA_HD Cx<T> bw_twobody(double ma, double mb, double ffr, T m_r, T g_r, double mpp, int LR) {

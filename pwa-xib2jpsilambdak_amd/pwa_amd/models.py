@@ -381,6 +381,32 @@ def config_spline(seed=5, nbins=40) -> Model:
     return m
 
 
+def config_revised(seed=6, nbins=40) -> Model:
+    """Lambda* in two revised K-matrix classes (type REVISEDKMATRIX): 1/2- with
+    Lambda(1405) below the K- p threshold, Lambda(1670), Lambda(1800) and a
+    non-resonant member; 3/2- with Lambda(1520), Lambda(1690).  Beside them
+    6 Breit-Wigner Lambda*, 3 smeared Pc Breit-Wigners and 1 Zcs.  Member
+    masses and widths float (the non-resonant member's mass stays fixed)."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    _nominal_smearing(d, nbins)
+    rk = abi.PWA_REVISEDKMATRIX
+    res = [lambda_res(rng, 1, -1, 1.405, 0.05, rk), lambda_res(rng, 1, -1, 1.67, 0.03, rk),
+           lambda_res(rng, 1, -1, 1.80, 0.20, rk), lambda_res(rng, 1, -1, -1.0, 0.4, rk),
+           lambda_res(rng, 3, -1, 1.5195, 0.0156, rk), lambda_res(rng, 3, -1, 1.69, 0.06, rk)]
+    for r in res:
+        r.floating[15] = True
+        r.floating[14] = r.params[14] > 0
+    res += [lambda_res(rng, s2, p, rng.uniform(1.60, 2.40), rng.uniform(0.05, 0.30))
+            for s2, p in [(1, 1), (3, 1), (5, -1), (5, 1), (7, 1), (9, 1)]]
+    for m0, (s2, p) in NOMINAL_PC:
+        res.append(pc_res(rng, s2, p, m0, rng.uniform(0.005, 0.03)))
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    m = assemble(d, res, "revised: 6 revised-K-matrix + 6 BW Lambda* + 3 Pc + 1 Zcs")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
     """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
     m = np.asarray(mjpsip)

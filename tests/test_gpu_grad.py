@@ -98,6 +98,12 @@ def test_grad_kmatrix():
     _check_fd(models.config_kmatrix(3), 400, 800, 43)
 
 
+def test_grad_lambda_revised_kmatrix():
+    """Masses and widths of revised K-matrix members: every member's
+    lineshape depends on all of them (k_adj_rkm)."""
+    _check_fd(models.config_revised(6), 400, 800, 45)
+
+
 def test_grad_lambda_spline():
     """Knot-value gradient of the Lambda* splines (cardinal basis from the
     slope matrix) and the rest of the spline model."""

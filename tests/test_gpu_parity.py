@@ -100,3 +100,8 @@ def test_lambda_spline_smeared_pc():
     evaluated per event by k_lsgen) with smeared Pc Breit-Wigners."""
     _check(models.config_spline(5), 1500, 3000, 25)
 
+
+def test_lambda_revised_kmatrix():
+    """Two revised K-matrix classes of Lambda* (one with Lambda(1405) below
+    threshold and a non-resonant member) with smeared Pc Breit-Wigners."""
+    _check(models.config_revised(6), 1500, 3000, 26)

@@ -62,7 +62,7 @@ def test_helicity_couplings_reference_preset_all_spins():
 def test_invalid_models_rejected():
     d = models.base_desc()
     d.n_lambda = 1
-    d.type[0] = 360  # revised K-matrix: not supported -> error, no fallback
+    d.type[0] = 2  # FLATTE: no lineshape in the reference's switch -> error, no fallback
     d.spin2[0] = 1
     d.parity[0] = 1
     n = C.c_int32()

@@ -298,10 +298,11 @@ def test_triangle_pc_lineshape_is_cusp_times_vertex():
 
 
 # --------------------------------------------------------------- library
-@pytest.mark.parametrize("case", ["lambda_spline", "pc_spline", "revised_km", "triangle"])
+@pytest.mark.parametrize("case", ["lambda_spline", "pc_spline", "revised_km", "pc_revised_km", "triangle"])
 def test_library_layout_and_acceptance(case):
     """The library counts every type's parameters as the oracle does; it
-    evaluates Lambda* splines and rejects the other types (PWA_E_MODEL)."""
+    evaluates Lambda* splines and revised K-matrix members and rejects the
+    Pc-chain forms and the triangle (PWA_E_MODEL)."""
     rng = np.random.default_rng(13)
     if case == "lambda_spline":
         r = models.lambda_spline_res(rng, 1, -1, np.linspace(1.45, 2.5, abi.PWA_NUMSP), np.ones(abi.PWA_NUMSP, complex))
@@ -309,6 +310,8 @@ def test_library_layout_and_acceptance(case):
         r = models.pc_spline_res(rng, 1, -1, np.linspace(4.05, 4.55, abi.PWA_NUMSPZ), np.ones(abi.PWA_NUMSPZ, complex))
     elif case == "revised_km":
         r = models.lambda_res(rng, 1, -1, 1.52, 0.02, abi.PWA_REVISEDKMATRIX)
+    elif case == "pc_revised_km":
+        r = models.pc_res(rng, 1, -1, 4.45, 0.02, abi.PWA_REVISEDKMATRIX)
     else:
         r = models.pc_res(rng, 1, -1, 4.2, 0.05, abi.PWA_TRIANGLE)
     m = models.assemble(models.base_desc(), [models.lambda_res(rng, 3, -1, 1.52, 0.016), r])
@@ -316,7 +319,7 @@ def test_library_layout_and_acceptance(case):
     assert _lib.lib().pwa_fit_param_count(C.byref(m.desc), C.byref(n)) == abi.PWA_OK
     assert n.value == len(m.params) == oracle.lib().orc_fit_param_count(oracle.dref(m.desc))
     rc = _lib.lib().pwa_helicity_couplings(C.byref(m.desc), None, None, C.byref(n))
-    if case == "lambda_spline":
+    if case in ("lambda_spline", "revised_km"):
         assert rc == abi.PWA_OK
         got = _lib.helicity_couplings(m.desc, m.params)
         ref = oracle.build_amp(m.desc, m.params)["vamp"]
