@@ -235,7 +235,7 @@ __device__ __forceinline__ void chain_ls_generic(const DevStatic* __restrict__ S
 // non-resonant members, Mt = K-matrix amplitude or 1) or Breit-Wigner x
 // vertex, its (m0, Gamma0) derivatives and the vertex factor Vx.  FULL as for
 // pc_lineshapes.
-template <bool FULL>
+template <int FULL>
 __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int rr,
                                            const ChainKin& kin, LsEv& e, double m, C64 Mt, C64& L, C64& d0,
                                            C64& d1, double& Vx) {
@@ -261,6 +261,8 @@ __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, cons
         const D2 m0 = D2::var(D->m0[rr], 0), g0 = D2::var(D->g0[rr], 1);
         Cx<D2> Ls(D2(Mt.re), D2(Mt.im));
         if (S->type[rr] == PWA_BREITWIGNER) Ls = bw_twobody<D2>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, g0, m, S->L[rr]);
+        if constexpr (FULL >= 2)
+            if (S->type[rr] == PWA_TRIANGLE) Ls = cusp_amp<D2>(S->P, m0, g0, m);
         const D2 Vv = vertex_pc<D2>(S->P, S->spin2[rr], m, m0);
         Vx = Vv.v;
         Ls = Ls * Vv;

@@ -319,10 +319,12 @@ __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
 }
 
 // Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.
-// FULL: the model has K-matrix members or Pc lineshapes outside the
+// FULL >= 1: the model has K-matrix members or Pc lineshapes outside the
 // factorised range this evaluation (else only factorised Breit-Wigner /
-// vertex terms occur and the other code is left out).
-template <int NZ, bool FULL = true>
+// vertex terms occur and the other code is left out); FULL = 2: also
+// triangle-cusp resonances (a code path of its own: the cusp's 100-point
+// integral would otherwise weigh on the registers of the K-matrix variant).
+template <int NZ, int FULL = 1>
 __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                               double m, C64 L[NZ]) {
     const int nR = S->nR;
@@ -354,6 +356,8 @@ __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, c
             const double m0 = D->m0[rr];
             if (S->type[rr] == PWA_BREITWIGNER)
                 L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);
+            if constexpr (FULL >= 2)
+                if (S->type[rr] == PWA_TRIANGLE) L[r] = cusp_amp<double>(S->P, m0, D->g0[rr], m);
             L[r] = L[r] * vertex_pc<double>(S->P, S->spin2[rr], m, m0);
         }
     }

@@ -151,7 +151,8 @@ struct StageArgs {
     int nblocks;          // blocks of the reducing kernels (sum of the segments' nblk)
     int passes;           // max over segments of ceil(n / (nblk kBlock)): strided passes of a block
     bool mc, grad;
-    bool full;            // K-matrix members or generic Pc lineshapes present (full k_shape variant)
+    int full;             // 1: K-matrix members or generic Pc lineshapes present (full k_shape variant);
+                          // 2: also triangle-cusp resonances
     int noint;
     const DevStatic* S;
     const DevDyn* D;

@@ -176,8 +176,7 @@ int validate(const pwa_model_desc& d, std::string& err) {
         } else if (t == PWA_REVISEDKMATRIX) {
             if (r >= d.n_lambda) { err = "the revised K-matrix is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
         } else if (t == PWA_TRIANGLE) {
-            err = "the triangle-cusp lineshape is not evaluated";
-            return PWA_E_MODEL;
+            if (!pc) { err = "the triangle-cusp lineshape is a Pc-chain lineshape"; return PWA_E_MODEL; }
         } else if (t != PWA_BREITWIGNER) {
             err = "unsupported lineshape type " + std::to_string(t);
             return PWA_E_MODEL;
@@ -372,6 +371,11 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         D.g0[r] = ires[size - 4];
         const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
         D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
+        if (d.type[r] == PWA_TRIANGLE) {  // direct (dual) evaluation in the FULL Pc path
+            D.ls[r] = LsConst{};
+            D.ls[r].kind = LS_GENERIC;
+            D.ls[r].slot = -1;
+        }
         if (c.hs.defer[r]) {  // spline / revised K-matrix: evaluated per event by k_lsgen
             D.ls[r] = LsConst{};
             D.ls[r].kind = LS_GENERIC;
@@ -558,7 +562,9 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.passes = s.passes;
         a.mc = (k == PWA_MC) && !mc_as_data;
         a.grad = grad;
-        a.full = c->hs.km_nclass > 0 || c->pc_generic;
+        a.full = (c->hs.km_nclass > 0 || c->pc_generic) ? 1 : 0;
+        for (int r = c->desc.n_lambda; r < c->desc.n_lambda + c->desc.n_pc; ++r)
+            if (c->desc.type[r] == PWA_TRIANGLE) a.full = 2;
         a.sv.direct_ok = (c->desc.n_pc > 0 && !a.full) ? 1 : 0;
         a.noint = noint;
         a.S = c->dS;

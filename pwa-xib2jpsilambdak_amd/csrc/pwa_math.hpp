@@ -100,6 +100,15 @@ template <int N> PWA_HD Dual<N> dtanh(const Dual<N>& a) {
     return r;
 }
 template <int N> PWA_HD Dual<N> dfabs(const Dual<N>& a) { return a.v < 0 ? -a : a; }
+template <int N> PWA_HD Dual<N> dlog(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = log(a.v);
+    const double h = 1.0 / a.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+    return r;
+}
+PWA_HD double dlog(double a) { return log(a); }
 PWA_HD double dsqrt(double a) { return sqrt(a); }
 PWA_HD double dtanh(double a) { return tanh(a); }
 PWA_HD double dfabs(double a) { return fabs(a); }
@@ -434,6 +443,96 @@ PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRk
     }
     const T inv = 1.0 / (denorm * denorm + totG * totG);
     return Cx<T>(factor * denorm * inv, factor * totG * inv);
+}
+
+// ------------------------------------------------- triangle cusp (Pc chain)
+// Argument of x + iy as the reference's complex log takes it, acos(x / r)
+// with the sign of y (RooTDalitz/cuComplex.h:658-667); a dual's derivative is
+// that of the argument, (x dy - y dx) / r^2.
+PWA_HD double arg_rule(double x, double y) {
+    const double t = acos(x / sqrt(x * x + y * y));
+    return y < 0 ? -t : t;
+}
+template <int N> PWA_HD Dual<N> arg_rule(const Dual<N>& x, const Dual<N>& y) {
+    Dual<N> t;
+    const double r2 = x.v * x.v + y.v * y.v;
+    t.v = arg_rule(x.v, y.v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) t.d[i] = (x.v * y.d[i] - y.v * x.d[i]) / r2;
+    return t;
+}
+
+// Complex square root with the reference's branch rule (cuCsqrtf,
+// RooTDalitz/cuComplex.h:641-655): the principal root, or its conjugate when
+// Im < 0 and |Im/Re| <= 1e-3; zero inside |Re|, |Im| < 1e-6.  The value
+// keeps the reference's formula; a dual's derivative is dz / (2 sqrt z) of
+// the principal root (conjugated with it), not the derivative of the
+// formula, whose cos-based imaginary part cancels near the real axis.
+PWA_HD C64 csqrt_rule(const C64& x) {
+    if (fabs(x.re) < 0.000001 && fabs(x.im) < 0.000001) return C64(0.0, 0.0);
+    const double radius = sqrt(x.re * x.re + x.im * x.im);
+    const double cosA = x.re / radius;
+    const double re = sqrt(radius * (cosA + 1.0) / 2.0);
+    double im = sqrt(radius * (1.0 - cosA) / 2.0);
+    if (__builtin_signbit(x.im) && fabs(x.im / x.re) > 0.001) im = -im;
+    return C64(re, im);
+}
+template <int N>
+PWA_HD Cx<Dual<N>> csqrt_rule(const Cx<Dual<N>>& x) {
+    const C64 v = csqrt_rule(C64(x.re.v, x.im.v));
+    Cx<Dual<N>> r(Dual<N>(v.re), Dual<N>(v.im));
+    if (v.re == 0.0 && v.im == 0.0) return r;
+    // principal root p and whether the rule returned its conjugate
+    const bool cj = __builtin_signbit(x.im.v) && !(v.im < 0.0);
+    const C64 p(v.re, cj ? -v.im : v.im);
+    const C64 q = cinv(C64(2.0 * p.re, 2.0 * p.im));
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const C64 dp = C64(x.re.d[i], x.im.d[i]) * q;
+        r.re.d[i] = dp.re;
+        r.im.d[i] = cj ? -dp.im : dp.im;
+    }
+    return r;
+}
+
+// a / b, and atan(z) = -i/2 log((1 + iz) / (1 - iz)) with the log above (cuCatanf).
+template <class T>
+PWA_HD Cx<T> cdiv(const Cx<T>& a, const Cx<T>& b) {
+    const T s = 1.0 / (b.re * b.re + b.im * b.im);
+    return Cx<T>((a.re * b.re + a.im * b.im) * s, (a.im * b.re - a.re * b.im) * s);
+}
+template <class T>
+PWA_HD Cx<T> catan_rule(const Cx<T>& z) {
+    const Cx<T> w = cdiv(Cx<T>(1.0 - z.im, z.re), Cx<T>(1.0 + z.im, 0.0 - z.re));
+    const T lr = dlog(dsqrt(w.re * w.re + w.im * w.im));
+    const T th = arg_rule(w.re, w.im);
+    return Cx<T>(0.5 * th, -0.5 * lr);
+}
+
+// GetCusp_AMP (src/cu_DPFPropogator.cu:1223-1343): triangle loop with a chi_c1
+// exchange in m(J/psi p) = mpp; 100-point midpoint rule over the Feynman
+// parameter alpha of (1/r)[atan(b/r) - atan((b + 2(alpha - 1))/r)],
+// r = sqrt(4c - b^2), divided by mpp^2.
+template <class T>
+PWA_HD Cx<T> cusp_amp(const Masses& P, T m_r, T g_r, double mpp) {
+    const double m_chic1 = 3.51067;  // src/cu_DPFPropogator.cu:15
+    const double ip2 = 1.0 / mpp / mpp;
+    const double kb = P.m_lb * P.m_lb - P.m_k * P.m_k - mpp * mpp, kb0 = P.m_p * P.m_p - m_chic1 * m_chic1;
+    // m1^2 = (m_r - i Gamma/2)^2
+    const T m1r = m_r * m_r - 0.25 * g_r * g_r, m1i = 0.0 - m_r * g_r;
+    const double m32 = P.m_p * P.m_p, mk2 = P.m_k * P.m_k;
+    Cx<T> sum(T(0.0), T(0.0));
+    for (int i = 0; i < 100; ++i) {
+        const double al = i * 0.01 + 0.01 / 2;
+        const Cx<T> b(T(ip2 * (al * kb + kb0) + 1.0), T(-0.001));
+        const Cx<T> c((al * (m1r - m32) + m32 - al * (1.0 - al) * mk2) * ip2, (al * m1i) * ip2);
+        const Cx<T> r = csqrt_rule(Cx<T>(4.0 * c.re - (b.re * b.re - b.im * b.im), 4.0 * c.im - 2.0 * b.re * b.im));
+        const Cx<T> a1 = catan_rule(cdiv(b, r));
+        const Cx<T> a2 = catan_rule(cdiv(Cx<T>(b.re + 2.0 * (al - 1.0), b.im), r));
+        const Cx<T> v = cdiv(a1 - a2, r);
+        sum = sum + Cx<T>(0.01 * v.re, 0.01 * v.im);
+    }
+    return Cx<T>(sum.re * ip2, sum.im * ip2);
 }
 
 // Breit-Wigner of a two-body system (ma, mb): the Pc (J/psi p) and Zcs (J/psi K) form.

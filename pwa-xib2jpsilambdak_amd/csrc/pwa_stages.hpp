@@ -110,8 +110,8 @@ __device__ __forceinline__ void event_terms(const DevStatic* __restrict__ S, dou
 // ----------------------------------------------------------- k_shape
 // Pc lineshapes over the event's sub-bins and the per-event likelihood term.
 // FULL: K-matrix members or Pc lineshapes outside the factorised range are
-// present this evaluation (pc_lineshapes).
-template <int NZ, bool MC, bool GRAD, bool FULL>
+// present this evaluation (1), and triangle-cusp resonances (2) (pc_lineshapes).
+template <int NZ, bool MC, bool GRAD, int FULL>
 __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev, int64_t ld, SegView sv,
                                                  const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                  const double* __restrict__ pcs, double* __restrict__ amp,
@@ -543,16 +543,21 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
         hipLaunchKernelGGL((k_shape<NZ, MC, GR, FL>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S,  \
                            a.D, pcs, amp, awp, a.fx, a.part, a.noint, G, NGp, a.gpart, a.rpb);                      \
     } while (0)
-    if (a.full) {
-        if (a.mc && a.grad) PWA_SHAPE(true, true, true);
-        else if (a.mc) PWA_SHAPE(true, false, true);
-        else if (a.grad) PWA_SHAPE(false, true, true);
-        else PWA_SHAPE(false, false, true);
+    if (a.full == 2) {
+        if (a.mc && a.grad) PWA_SHAPE(true, true, 2);
+        else if (a.mc) PWA_SHAPE(true, false, 2);
+        else if (a.grad) PWA_SHAPE(false, true, 2);
+        else PWA_SHAPE(false, false, 2);
+    } else if (a.full) {
+        if (a.mc && a.grad) PWA_SHAPE(true, true, 1);
+        else if (a.mc) PWA_SHAPE(true, false, 1);
+        else if (a.grad) PWA_SHAPE(false, true, 1);
+        else PWA_SHAPE(false, false, 1);
     } else {
-        if (a.mc && a.grad) PWA_SHAPE(true, true, false);
-        else if (a.mc) PWA_SHAPE(true, false, false);
-        else if (a.grad) PWA_SHAPE(false, true, false);
-        else PWA_SHAPE(false, false, false);
+        if (a.mc && a.grad) PWA_SHAPE(true, true, 0);
+        else if (a.mc) PWA_SHAPE(true, false, 0);
+        else if (a.grad) PWA_SHAPE(false, true, 0);
+        else PWA_SHAPE(false, false, 0);
     }
 #undef PWA_SHAPE
     if ((e = hipGetLastError()) != hipSuccess) return e;

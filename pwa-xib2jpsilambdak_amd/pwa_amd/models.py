@@ -407,6 +407,26 @@ def config_revised(seed=6, nbins=40) -> Model:
     return m
 
 
+def config_triangle(seed=7, nbins=40) -> Model:
+    """A triangle-cusp Pc (type TRIANGLE, m0 = 4.3 GeV, width 0.2 GeV: a range
+    in which the reference's closed form equals the loop integral) beside two
+    smeared Pc Breit-Wigners, 8 Breit-Wigner Lambda* and 1 Zcs.  The cusp's
+    mass and width float."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    _nominal_smearing(d, nbins)
+    res = [lambda_res(rng, s2, p, rng.uniform(1.50, 2.45), rng.uniform(0.02, 0.30))
+           for s2, p in [(1, -1), (3, -1), (3, 1), (5, -1), (5, 1), (7, 1), (1, 1), (9, 1)]]
+    res += [pc_res(rng, 1, -1, 4.312, 0.0098), pc_res(rng, 3, -1, 4.457, 0.0064)]
+    cusp = pc_res(rng, 1, 1, 4.3, 0.2, abi.PWA_TRIANGLE)
+    cusp.floating[12] = cusp.floating[13] = True
+    res.append(cusp)
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    m = assemble(d, res, "triangle: 8 Lambda* + 2 Pc BW + 1 Pc triangle cusp + 1 Zcs")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
     """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
     m = np.asarray(mjpsip)

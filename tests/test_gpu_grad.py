@@ -49,10 +49,10 @@ def _fd(fun, p, idx, five_point=True):
     return g
 
 
-def _check_fd(model, nd, nm, seed, n_oracle=8):
+def _check_fd(model, nd, nm, seed, n_oracle=8, exclude=()):
     dcols, sw, ds, mcols, wmc, ms = _setup(model, nd, nm, seed)
     p = model.params.copy()
-    idx = np.flatnonzero(model.float_mask)
+    idx = np.array([j for j in np.flatnonzero(model.float_mask) if j not in set(exclude)], dtype=np.int64)
     print(f"[grad] {model.name} nd={nd} nm={nm} nfloat={len(idx)}", flush=True)
     with Context(model.desc) as c:
         c.upload(abi.PWA_DATA, dcols, sw, ds)
@@ -84,6 +84,7 @@ def _check_fd(model, nd, nm, seed, n_oracle=8):
     erro = np.abs(g[sub] - gor)
     print(f"[grad] oracle-FD subset max abs err={erro.max():.3e}", flush=True)
     assert np.all(erro <= 1e-6 * scale + 2e-5 * np.abs(gor))
+    return g
 
 
 def test_grad_small():
@@ -102,6 +103,25 @@ def test_grad_lambda_revised_kmatrix():
     """Masses and widths of revised K-matrix members: every member's
     lineshape depends on all of them (k_adj_rkm)."""
     _check_fd(models.config_revised(6), 400, 800, 45)
+
+
+def test_grad_pc_triangle_cusp():
+    """The triangle-cusp model's gradient.  The cusp's own mass and width are
+    left out of the finite-difference comparison: the reference's complex
+    square root flips branch at |Im/Re| = 1e-3, so the likelihood jumps by
+    ~1e-5 wherever one of the ~1e5 loop-integral points crosses that line,
+    and a finite-difference step over the sample crosses some.  Those two
+    derivatives (dual numbers through the loop integral) are checked on the
+    host against differences of the cusp itself, away from the jumps
+    (tests/test_kmatrix_adjoint_cpu.py); here they must be finite and the
+    rest of the gradient must match."""
+    m = models.config_triangle(7)
+    r = m.desc.n_res - 2  # the cusp, before the Zcs
+    assert m.desc.type[r] == abi.PWA_TRIANGLE
+    o = m.desc.param_offset(r)
+    g = _check_fd(m, 300, 600, 46, exclude=(o + 12, o + 13))
+    print(f"[grad] cusp dNLL/dm0 {g[o + 12]:.6e} dNLL/dwidth {g[o + 13]:.6e}", flush=True)
+    assert np.isfinite(g[o + 12]) and np.isfinite(g[o + 13]) and g[o + 12] != 0.0 and g[o + 13] != 0.0
 
 
 def test_grad_lambda_spline():

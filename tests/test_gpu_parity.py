@@ -105,3 +105,10 @@ def test_lambda_revised_kmatrix():
     """Two revised K-matrix classes of Lambda* (one with Lambda(1405) below
     threshold and a non-resonant member) with smeared Pc Breit-Wigners."""
     _check(models.config_revised(6), 1500, 3000, 26)
+
+
+def test_pc_triangle_cusp():
+    """A triangle-cusp Pc (its 100-point loop integral per sub-bin, the
+    reference's complex branch rules) with smeared Pc Breit-Wigners."""
+    _check(models.config_triangle(7), 1000, 2000, 27)
+

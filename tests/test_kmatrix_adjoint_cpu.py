@@ -1,7 +1,8 @@
-"""Host-side regression check of the K-matrix reverse pass (the same header
-the gradient kernel uses, compiled for the CPU by build.sh): adjoints of the
-K-matrix couplings, background terms and pole masses against central finite
-differences of the forward K-matrix."""
+"""Host-side checks of derivative code the gradient kernels use (the same
+headers, compiled for the CPU by build.sh): the K-matrix reverse pass
+(adjoints of the couplings, background terms and pole masses) and the dual
+numbers of the triangle-cusp and revised K-matrix lineshapes, against central
+finite differences of the forward evaluations."""
 import os
 import subprocess
 
@@ -16,3 +17,21 @@ def test_kmatrix_adjoint_matches_finite_differences():
     out = subprocess.run([EXE], capture_output=True, text=True, check=True).stdout
     worst = float(out.strip().splitlines()[-1].split()[-1])
     assert worst < 2e-6, out[-2000:]
+
+
+DUAL = os.path.join(ROOT, "pwa-xib2jpsilambdak_amd", "_build", "lineshape_dual_check")
+
+
+@pytest.mark.skipif(not os.path.exists(DUAL), reason="lineshape_dual_check not built (run build.sh)")
+def test_cusp_and_revised_kmatrix_duals_match_finite_differences():
+    """Mass and width derivatives of the triangle cusp (m0, width, m(J/psi p)
+    over a grid) and of every member of a revised K-matrix class with respect
+    to every member's mass and width; steps that straddle one of the cusp's
+    branch-rule jumps are skipped (reported)."""
+    out = subprocess.run([DUAL], capture_output=True, text=True, check=True).stdout
+    lines = out.strip().splitlines()
+    worst = float(lines[-1].split()[-1])
+    compared = int(lines[-2].split()[1].rstrip(","))
+    assert compared > 1500, out[-2000:]
+    assert worst < 1e-5, out[-2000:]
+

@@ -301,8 +301,9 @@ def test_triangle_pc_lineshape_is_cusp_times_vertex():
 @pytest.mark.parametrize("case", ["lambda_spline", "pc_spline", "revised_km", "pc_revised_km", "triangle"])
 def test_library_layout_and_acceptance(case):
     """The library counts every type's parameters as the oracle does; it
-    evaluates Lambda* splines and revised K-matrix members and rejects the
-    Pc-chain forms and the triangle (PWA_E_MODEL)."""
+    evaluates Lambda* splines, Lambda* revised K-matrix members and the Pc
+    triangle cusp, and rejects the Pc-chain spline and revised K-matrix
+    (PWA_E_MODEL)."""
     rng = np.random.default_rng(13)
     if case == "lambda_spline":
         r = models.lambda_spline_res(rng, 1, -1, np.linspace(1.45, 2.5, abi.PWA_NUMSP), np.ones(abi.PWA_NUMSP, complex))
@@ -319,7 +320,7 @@ def test_library_layout_and_acceptance(case):
     assert _lib.lib().pwa_fit_param_count(C.byref(m.desc), C.byref(n)) == abi.PWA_OK
     assert n.value == len(m.params) == oracle.lib().orc_fit_param_count(oracle.dref(m.desc))
     rc = _lib.lib().pwa_helicity_couplings(C.byref(m.desc), None, None, C.byref(n))
-    if case in ("lambda_spline", "revised_km"):
+    if case in ("lambda_spline", "revised_km", "triangle"):
         assert rc == abi.PWA_OK
         got = _lib.helicity_couplings(m.desc, m.params)
         ref = oracle.build_amp(m.desc, m.params)["vamp"]
