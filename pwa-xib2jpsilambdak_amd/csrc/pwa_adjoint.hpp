@@ -258,6 +258,15 @@ __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, cons
         return;
     }
     if constexpr (FULL) {
+        if constexpr (FULL >= 2)
+            if (S->type[rr] == PWA_MODELINDEPENDENT || S->type[rr] == PWA_REVISEDKMATRIX) {
+                // value only: k_adj_pcx emits their parameter rows
+                L = pc_exotic_value(S, D, rr, m);
+                d0 = C64(0.0, 0.0);
+                d1 = C64(0.0, 0.0);
+                Vx = 0.0;
+                return;
+            }
         const D2 m0 = D2::var(D->m0[rr], 0), g0 = D2::var(D->g0[rr], 1);
         Cx<D2> Ls(D2(Mt.re), D2(Mt.im));
         if (S->type[rr] == PWA_BREITWIGNER) Ls = bw_twobody<D2>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, g0, m, S->L[rr]);
@@ -363,7 +372,7 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
 __device__ __forceinline__ void rkm_rows(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                          const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
                                          double a2, const GradLayout& G, Emitter& em) {
-    for (int c = 0; c < S->rkm_ncls; ++c) {
+    for (int c = 0; c < S->rkm_nlam; ++c) {
         const int n = S->rkm_nmem[c];
         double mv[kRkmMem], gv[kRkmMem];
         uint32_t Lpk = 0, Rpk = 0;  // orbital momenta, resonance indices: byte q of member q

@@ -318,12 +318,53 @@ __device__ __forceinline__ C64 select8(const C64 v[8], int idx) {
     return b2 ? c1 : c0;
 }
 
+// v[q] of an NZ-entry register array by a tree of selects (q < NZ <= 8).
+template <int NZ>
+__device__ __forceinline__ C64 select_nz(const C64 (&v)[NZ], int q) {
+    auto at = [&](int k) { return v[k < NZ ? k : NZ - 1]; };
+    const bool b0 = q & 1, b1 = q & 2, b2 = q & 4;
+    const C64 a0 = b0 ? at(1) : at(0), a1 = b0 ? at(3) : at(2), a2 = b0 ? at(5) : at(4), a3 = b0 ? at(7) : at(6);
+    const C64 c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2;
+    return b2 ? c1 : c0;
+}
+
+// Pc spline (type MODELINDEPENDENT: ModelIndependentWave, src/cu_DPFPropogator.cu:1170-1221,
+// zero outside [m_p + m_J/psi - 2 MeV, 4.6 GeV], vertex at m0 = -0.5) or Pc
+// revised K-matrix member (Revised_KMAMP with the Pc-chain masses, vertex at
+// its own mass) rr at m(J/psi p) = m.
+__device__ __forceinline__ C64 pc_exotic_value(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int rr,
+                                               double m) {
+    if (S->type[rr] == PWA_MODELINDEPENDENT) {
+        const SplTabZ& T = D->splz[S->splz_of[rr]];
+        if (!(m >= T.xmin && m <= T.xmax)) return C64(0.0, 0.0);
+        const Hermite H = spl_hermite<PWA_NUMSPZ>(T.x, m);
+        const int k = H.k;
+        const double pr = T.yr[k] * H.h00 + T.yr[k + 1] * H.h01 + T.sr[k] * H.h10 + T.sr[k + 1] * H.h11;
+        const double pi = T.yi[k] * H.h00 + T.yi[k + 1] * H.h01 + T.si[k] * H.h10 + T.si[k + 1] * H.h11;
+        return C64(pr, pi) * vertex_pc<double>(S->P, S->spin2[rr], m, -0.5);
+    }
+    const int c = S->rkm_cls[rr], n = S->rkm_nmem[c];
+    double mv[kRkmMem], gv[kRkmMem];
+    uint32_t Lpk = 0;
+    int self = 0;
+#pragma unroll
+    for (int q = 0; q < kRkmMem; ++q) {
+        const int rq = S->rkm_mem[c][q < n ? q : 0];
+        mv[q] = D->m0[rq];
+        gv[q] = D->g0[rq];
+        Lpk |= (uint32_t)(S->L[rq] & 0xff) << (8 * q);
+        if (q < n && rq == rr) self = q;
+    }
+    return rkm_amp<double>(S->P, true, mv, gv, Lpk, n, self, m) * vertex_pc<double>(S->P, S->spin2[rr], m, D->m0[rr]);
+}
+
 // Pc lineshapes L_r(m) (lineshape x Lambda_b vertex barrier) for all NZ Pc.
 // FULL >= 1: the model has K-matrix members or Pc lineshapes outside the
 // factorised range this evaluation (else only factorised Breit-Wigner /
 // vertex terms occur and the other code is left out); FULL = 2: also
-// triangle-cusp resonances (a code path of its own: the cusp's 100-point
-// integral would otherwise weigh on the registers of the K-matrix variant).
+// triangle-cusp, spline or revised K-matrix Pc resonances (a code path of its
+// own: their code would otherwise weigh on the registers of the K-matrix
+// variant).
 template <int NZ, int FULL = 1>
 __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                               double m, C64 L[NZ]) {
@@ -353,6 +394,11 @@ __device__ __forceinline__ void pc_lineshapes(const DevStatic* __restrict__ S, c
             continue;
         }
         if constexpr (FULL) {
+            if constexpr (FULL >= 2)
+                if (S->type[rr] == PWA_MODELINDEPENDENT || S->type[rr] == PWA_REVISEDKMATRIX) {
+                    L[r] = pc_exotic_value(S, D, rr, m);
+                    continue;
+                }
             const double m0 = D->m0[rr];
             if (S->type[rr] == PWA_BREITWIGNER)
                 L[r] = bw_twobody<double>(S->P.m_p, S->P.m_jpsi, S->P.ffr, m0, D->g0[rr], m, S->L[rr]);

@@ -47,14 +47,19 @@ enum Plane {
 // knot values y and not-a-knot slopes s, and the slope matrix sm (s = sm y, a
 // function of the positions only) that gives the knot gradient.  The cubic on
 // [x_k, x_k+1] is the Hermite form of (y_k, y_k+1, s_k, s_k+1).
-constexpr int kMaxSpline = 2;
-struct SplTab {
-    double x[PWA_NUMSP];
-    double yr[PWA_NUMSP], yi[PWA_NUMSP];
-    double sr[PWA_NUMSP], si[PWA_NUMSP];
-    double sm[PWA_NUMSP * PWA_NUMSP];  // row k: d s_k / d y_j
-    double xmin, xmax;                 // zero outside [xmin, xmax]
+// Pc-chain splines (ModelIndependentWave, :1170-1221) alike with PWA_NUMSPZ knots.
+constexpr int kMaxSpline = 2;   // Lambda* spline resonances
+constexpr int kMaxSplineZ = 2;  // Pc spline resonances
+template <int N>
+struct SplTabN {
+    double x[N];
+    double yr[N], yi[N];
+    double sr[N], si[N];
+    double sm[N * N];  // row k: d s_k / d y_j
+    double xmin, xmax; // zero outside [xmin, xmax]
 };
+using SplTab = SplTabN<PWA_NUMSP>;
+using SplTabZ = SplTabN<PWA_NUMSPZ>;
 
 // Static model tables, uploaded once per context.
 struct DevStatic {
@@ -70,9 +75,12 @@ struct DevStatic {
     int32_t spl_of[PWA_MAX_RES];    // spline table of a Lambda* spline resonance, else -1
     int32_t nspl;
     int32_t spl_res[kMaxSpline];    // resonance of each spline table
-    // Lambda* revised K-matrix classes: members (resonance indices, list order)
+    int32_t splz_of[PWA_MAX_RES];   // spline table of a Pc spline resonance, else -1
+    int32_t nsplz;
+    // revised K-matrix classes (2J, P, chain): members (resonance indices, list order)
     int32_t rkm_cls[PWA_MAX_RES];   // class of a member, else -1
-    int32_t rkm_ncls;
+    int32_t rkm_ncls;               // Lambda* classes first: [0, rkm_nlam), then Pc classes
+    int32_t rkm_nlam;
     int32_t rkm_nmem[kRkmCls];
     int32_t rkm_mem[kRkmCls][kRkmMem];
     // Lambda* resonances whose parameter gradient a second pass emits
@@ -111,6 +119,7 @@ struct DevDyn {
     double km_dm2m[PWA_KM_NCLASS][PWA_KM_NPOLE];
     double km_dm2g[PWA_KM_NCLASS][PWA_KM_NPOLE];
     SplTab spl[kMaxSpline];
+    SplTabZ splz[kMaxSplineZ];
 };
 
 // Per-block reduction record: compensated sums (hi, lo).
@@ -161,7 +170,8 @@ struct StageArgs {
     double* fx;           // per-event output (device order) or null
     Partial* part;        // nblocks partial sums
     int ncpl, nres, ng;   // gradient layout
-    int nrkm;             // revised K-matrix classes (k_adj_rkm runs after k_adj_lam)
+    int nrkm;             // Lambda* revised K-matrix classes (k_adj_rkm runs after k_adj_lam)
+    int pcx;              // Pc splines or Pc revised K-matrix classes (k_adj_pcx runs after k_shape)
     double* gpart;        // gradient partial rows: block b of k_shape or k_pcdirect / k_adj_lam / k_adj_pc
                           // -> rows rpb b + 0 / 1 / 2
     int rpb;              // gradient rows per block (2, or 3 with Pc-chain resonances)

@@ -158,7 +158,7 @@ int validate(const pwa_model_desc& d, std::string& err) {
     for (int s = 0; s < d.n_schemes; ++s)
         if (d.scheme_nbins[s] < 1 || d.scheme_nbins[s] > PWA_MAX_SUBBINS) { err = "scheme_nbins out of range"; return PWA_E_MODEL; }
     int members[4] = {0, 0, 0, 0};
-    int nspl = 0;
+    int nspl = 0, nsplz = 0;
     for (int r = 0; r < n; ++r) {
         const int t = d.type[r];
         if (d.parity[r] != 1 && d.parity[r] != -1) { err = "parity must be +-1"; return PWA_E_MODEL; }
@@ -171,10 +171,11 @@ int validate(const pwa_model_desc& d, std::string& err) {
         } else if (t == PWA_NONRESONANT) {
             if (zcs) { err = "non-resonant Zcs is not supported"; return PWA_E_MODEL; }
         } else if (t == PWA_MODELINDEPENDENT) {
-            if (r >= d.n_lambda) { err = "the spline lineshape is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
-            if (++nspl > kMaxSpline) { err = "at most 2 spline resonances are supported"; return PWA_E_MODEL; }
+            if (zcs) { err = "the spline lineshape is evaluated for Lambda* and Pc resonances only"; return PWA_E_MODEL; }
+            if (!pc && ++nspl > kMaxSpline) { err = "at most 2 Lambda* spline resonances are supported"; return PWA_E_MODEL; }
+            if (pc && ++nsplz > kMaxSplineZ) { err = "at most 2 Pc spline resonances are supported"; return PWA_E_MODEL; }
         } else if (t == PWA_REVISEDKMATRIX) {
-            if (r >= d.n_lambda) { err = "the revised K-matrix is evaluated for Lambda* resonances only"; return PWA_E_MODEL; }
+            if (zcs) { err = "the revised K-matrix is evaluated for Lambda* and Pc resonances only"; return PWA_E_MODEL; }
         } else if (t == PWA_TRIANGLE) {
             if (!pc) { err = "the triangle-cusp lineshape is a Pc-chain lineshape"; return PWA_E_MODEL; }
         } else if (t != PWA_BREITWIGNER) {
@@ -184,11 +185,11 @@ int validate(const pwa_model_desc& d, std::string& err) {
     }
     for (int k = 0; k < 4; ++k)
         if (members[k] > 8) { err = "more than 8 members in a K-matrix class"; return PWA_E_MODEL; }
-    // revised K-matrix classes (2J, P) of the Lambda* chain
+    // revised K-matrix classes (chain, 2J, P) of the Lambda* and Pc chains
     int key[kRkmCls + 1], cnt[kRkmCls + 1], ncls = 0;
-    for (int r = 0; r < d.n_lambda; ++r) {
+    for (int r = 0; r < d.n_lambda + d.n_pc; ++r) {
         if (d.type[r] != PWA_REVISEDKMATRIX) continue;
-        const int k = d.spin2[r] * 2 + (d.parity[r] > 0 ? 1 : 0);
+        const int k = (r < d.n_lambda ? 0 : 64) + d.spin2[r] * 2 + (d.parity[r] > 0 ? 1 : 0);
         int c = 0;
         while (c < ncls && key[c] != k) ++c;
         if (c == ncls) {
@@ -219,20 +220,26 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
         S.ik[r] = -1;
         S.ikm[r] = -1;
         S.spl_of[r] = -1;
+        S.splz_of[r] = -1;
         S.rkm_cls[r] = -1;
-        if (d.type[r] == PWA_MODELINDEPENDENT && r < S.nR && S.nspl < kMaxSpline) {
+        const bool lam = r < S.nR, pcr = r >= S.nR && r < S.nR + S.nZ;
+        if (d.type[r] == PWA_MODELINDEPENDENT && lam && S.nspl < kMaxSpline) {
             S.spl_res[S.nspl] = r;
             S.spl_of[r] = S.nspl++;
             S.defer[r] = 1;
         }
-        if (d.type[r] == PWA_REVISEDKMATRIX && r < S.nR) {  // class = (2J, P), members in list order
+        if (d.type[r] == PWA_MODELINDEPENDENT && pcr && S.nsplz < kMaxSplineZ) S.splz_of[r] = S.nsplz++;
+        if (d.type[r] == PWA_REVISEDKMATRIX && (lam || pcr)) {  // class = (chain, 2J, P), members in list order
             int c = 0;
-            while (c < S.rkm_ncls && !(d.spin2[S.rkm_mem[c][0]] == d.spin2[r] && d.parity[S.rkm_mem[c][0]] == d.parity[r])) ++c;
+            while (c < S.rkm_ncls && !((S.rkm_mem[c][0] < S.nR) == lam && d.spin2[S.rkm_mem[c][0]] == d.spin2[r] &&
+                                       d.parity[S.rkm_mem[c][0]] == d.parity[r]))
+                ++c;
             if (c == S.rkm_ncls && S.rkm_ncls < kRkmCls) S.rkm_nmem[S.rkm_ncls++] = 0;
             if (c < S.rkm_ncls && S.rkm_nmem[c] < kRkmMem) {
                 S.rkm_mem[c][S.rkm_nmem[c]++] = r;
                 S.rkm_cls[r] = c;
-                S.defer[r] = 1;
+                if (lam) S.defer[r] = 1;
+                if (lam) S.rkm_nlam = std::max(S.rkm_nlam, c + 1);
             }
         }
         if (d.type[r] == PWA_KMATRIX) {
@@ -332,14 +339,17 @@ bool spline_slope_matrix(int n, const double* x, double* sm) {
     return true;
 }
 
-// Spline table of Lambda* spline resonance r from its knot triples (Re, Im, x)
-// at offset 12 of its parameter list (MakeSpline, src/RooDalitzAmplitude.cxx:1721-1736).
-bool fill_spline(const pwa_model_desc& d, const double* ires, SplTab& T) {
-    const int n = PWA_NUMSP;
+// Spline table of a spline resonance from its N knot triples (Re, Im, x) at
+// offset noff of its parameter list: 12 for a Lambda*, 8 for a Pc (MakeSpline,
+// MakeSplineZ, src/RooDalitzAmplitude.cxx:1721-1750), zero outside
+// [xmin, xmax] (ModelIndependentWave3 / ModelIndependentWave).
+template <int N>
+bool fill_spline(const double* ires, int noff, double xmin, double xmax, SplTabN<N>& T) {
+    const int n = N;
     for (int i = 0; i < n; ++i) {
-        T.yr[i] = ires[12 + 3 * i];
-        T.yi[i] = ires[12 + 3 * i + 1];
-        T.x[i] = ires[12 + 3 * i + 2];
+        T.yr[i] = ires[noff + 3 * i];
+        T.yi[i] = ires[noff + 3 * i + 1];
+        T.x[i] = ires[noff + 3 * i + 2];
     }
     if (!spline_slope_matrix(n, T.x, T.sm)) return false;
     for (int k = 0; k < n; ++k) {
@@ -351,8 +361,8 @@ bool fill_spline(const pwa_model_desc& d, const double* ires, SplTab& T) {
         T.sr[k] = a;
         T.si[k] = b;
     }
-    T.xmin = d.m_p + d.m_k - 0.002;
-    T.xmax = d.m_lb - d.m_jpsi + 0.002;
+    T.xmin = xmin;
+    T.xmax = xmax;
     return true;
 }
 
@@ -371,15 +381,21 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         D.g0[r] = ires[size - 4];
         const int chain = r < d.n_lambda ? CH_LAMBDA : (r < d.n_lambda + d.n_pc ? CH_PC : CH_ZCS);
         D.ls[r] = ls_const(c.hs.P, chain, d.type[r], d.spin2[r], c.hs.L[r], D.m0[r], D.g0[r]);
-        if (d.type[r] == PWA_TRIANGLE) {  // direct (dual) evaluation in the FULL Pc path
+        if (chain == CH_PC && (d.type[r] == PWA_TRIANGLE || d.type[r] == PWA_MODELINDEPENDENT ||
+                               d.type[r] == PWA_REVISEDKMATRIX)) {  // direct evaluation in the FULL Pc path
             D.ls[r] = LsConst{};
             D.ls[r].kind = LS_GENERIC;
             D.ls[r].slot = -1;
+            if (c.hs.splz_of[r] >= 0 &&
+                !fill_spline<PWA_NUMSPZ>(ires, 8, d.m_p + d.m_jpsi - 0.002, 4.6, D.splz[c.hs.splz_of[r]]))
+                return -1;
         }
-        if (c.hs.defer[r]) {  // spline / revised K-matrix: evaluated per event by k_lsgen
+        if (c.hs.defer[r]) {  // Lambda* spline / revised K-matrix: evaluated per event by k_lsgen
             D.ls[r] = LsConst{};
             D.ls[r].kind = LS_GENERIC;
-            if (c.hs.spl_of[r] >= 0 && !fill_spline(d, ires, D.spl[c.hs.spl_of[r]])) return -1;
+            if (c.hs.spl_of[r] >= 0 &&
+                !fill_spline<PWA_NUMSP>(ires, 12, d.m_p + d.m_k - 0.002, d.m_lb - d.m_jpsi + 0.002, D.spl[c.hs.spl_of[r]]))
+                return -1;
         }
         if (D.ls[r].kind == LS_GENERIC && chain != CH_PC) D.ls[r].slot = ngen++;
         if (d.type[r] != PWA_KMATRIX) continue;
@@ -564,7 +580,10 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.grad = grad;
         a.full = (c->hs.km_nclass > 0 || c->pc_generic) ? 1 : 0;
         for (int r = c->desc.n_lambda; r < c->desc.n_lambda + c->desc.n_pc; ++r)
-            if (c->desc.type[r] == PWA_TRIANGLE) a.full = 2;
+            if (c->desc.type[r] == PWA_TRIANGLE || c->desc.type[r] == PWA_MODELINDEPENDENT ||
+                c->desc.type[r] == PWA_REVISEDKMATRIX)
+                a.full = 2;
+        a.pcx = (c->hs.nsplz > 0 || c->hs.rkm_ncls > c->hs.rkm_nlam) ? 1 : 0;
         a.sv.direct_ok = (c->desc.n_pc > 0 && !a.full) ? 1 : 0;
         a.noint = noint;
         a.S = c->dS;
@@ -578,7 +597,7 @@ int launch_all(pwa_ctx* c, const double* params, bool grad, bool want_fx, bool f
         a.ncpl = c->ncpl;
         a.nres = c->nres;
         a.ng = NG;
-        a.nrkm = c->hs.rkm_ncls;
+        a.nrkm = c->hs.rkm_nlam;
         a.gpart = grad ? c->gpart + R0[k] * NG : nullptr;
         a.rpb = rpb;
         const int li = c->nlaunch < pwa_ctx::kMaxLaunch ? c->nlaunch++ : -1;
@@ -777,6 +796,14 @@ void map_gradient(const pwa_ctx* c, const double* params, const std::vector<doub
             }
             continue;
         }
+        if (c->hs.splz_of[r] >= 0) {  // Pc spline knot values
+            const double* gs = &gk[o_spl + 2 * PWA_NUMSP * c->hs.nspl + 2 * PWA_NUMSPZ * c->hs.splz_of[r]];
+            for (int j = 0; j < PWA_NUMSPZ; ++j) {
+                grad[off + 8 + 3 * j] += gs[2 * j];
+                grad[off + 9 + 3 * j] += gs[2 * j + 1];
+            }
+            continue;
+        }
         grad[off + size - 5] += gk[o_res + 2 * r];
         if (d.type[r] != PWA_KMATRIX) {
             grad[off + size - 4] += gk[o_res + 2 * r + 1];
@@ -838,7 +865,8 @@ int pwa_create(const pwa_model_desc* model, int device, pwa_ctx** out) {
     if (bad(hipMalloc(&c->dS, sizeof(DevStatic)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dD, sizeof(DevDyn)))) return PWA_E_NOMEM;
     if (bad(hipMalloc(&c->dC, sizeof(C64) * std::max(c->ncpl, 1)))) return PWA_E_NOMEM;
-    c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams + 2 * PWA_NUMSP * c->hs.nspl;
+    c->NG = 2 * c->ncpl + 2 * c->nres + PWA_KM_NCLASS * kGradNClassParams + 2 * PWA_NUMSP * c->hs.nspl +
+            2 * PWA_NUMSPZ * c->hs.nsplz;
     if (ensure_groups(c, 2) != PWA_OK) {
         pwa_destroy(c);
         return PWA_E_NOMEM;

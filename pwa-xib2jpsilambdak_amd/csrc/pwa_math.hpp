@@ -405,14 +405,18 @@ PWA_HD T rkm_term(const Masses& P, T mq, T gq, int Lq, double mpp, double pR, T&
     return mq * gamma / (mq * mq - s);
 }
 
-// Member r of a class with members q < n (list order) at m(K p) = mpp;
-// member q's orbital momentum is byte q of Lpk (a packed word, not an int
-// array: the compiler turns a select over an int array into an indexed load).
+// Member r of a class with members q < n (list order) at mpp, the K- p mass
+// (pc false) or the J/psi p mass (pc true; the members' K-matrix terms keep
+// the K- p masses, as KMatrixTerm does); member q's orbital momentum is byte
+// q of Lpk (a packed word, not an int array: the compiler turns a select over
+// an int array into an indexed load).
 template <class T>
-PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRkmMem], uint32_t Lpk, int n, int r,
-                        double mpp) {
-    const double s = mpp * mpp, thr = P.m_p + P.m_k;
-    const double pR = breakup_d(mpp, P.m_p, P.m_k);
+PWA_HD Cx<T> rkm_amp(const Masses& P, bool pc, const T (&m)[kRkmMem], const T (&g)[kRkmMem], uint32_t Lpk, int n,
+                     int r, double mpp) {
+    const double m1 = pc ? P.m_jpsi : P.m_p, m2 = pc ? P.m_p : P.m_k, m3 = pc ? P.m_k : P.m_jpsi;
+    const double s = mpp * mpp, thr = m1 + m2, mmax = P.m_lb - m3;
+    const double pR = breakup_d(mpp, m1, m2);
+    const double pRk = breakup_d(mpp, P.m_p, P.m_k);
     const T mr = sel4(m, r), gr = sel4(g, r);
     const int Lr = (int)((Lpk >> (8 * r)) & 0xffu);
     const double mrv = val(mr);
@@ -421,9 +425,11 @@ PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRk
 #pragma unroll
     for (int q = 0; q < kRkmMem; ++q) {
         if (q >= n || q == r) continue;
-        totG = totG + rkm_term(P, m[q], g[q], (int)((Lpk >> (8 * q)) & 0xffu), mpp, pR, fsc) * denorm;
+        totG = totG + rkm_term(P, m[q], g[q], (int)((Lpk >> (8 * q)) & 0xffu), mpp, pRk, fsc) * denorm;
     }
-    const T pR0 = breakup(rkm_mreff(P, mr), P.m_p, P.m_k);
+    T mreff = mr;
+    if (!(mrv > thr)) mreff = (mrv >= 0) ? tanh_mass(mr, thr, mmax) : T(0.5 * (thr + mmax));
+    const T pR0 = breakup(mreff, m1, m2);
     const T FR = barrier_ratio(P.ffr, Lr, T(pR), pR0);
     T factor = FR * ipow(pR / pR0, Lr);
     if (mrv > 0) {
@@ -443,6 +449,44 @@ PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRk
     }
     const T inv = 1.0 / (denorm * denorm + totG * totG);
     return Cx<T>(factor * denorm * inv, factor * totG * inv);
+}
+template <class T>
+PWA_HD Cx<T> rkm_lambda(const Masses& P, const T (&m)[kRkmMem], const T (&g)[kRkmMem], uint32_t Lpk, int n, int r,
+                        double mpp) {
+    return rkm_amp<T>(P, false, m, g, Lpk, n, r, mpp);
+}
+
+// ------------------------------------------------------- spline evaluation
+// Knot interval of m by bisection (TSpline3 / ModelIndependentWave), stopping
+// at n - 2 (the reference's last-knot row is the last interval's cubic), and
+// the Hermite basis at t = (m - x_k) / h: value y_k h00 + y_k+1 h01 +
+// h (s_k h10 + s_k+1 h11).
+template <int N>
+PWA_HD int spl_interval_n(const double* x, double m) {
+    int lo = 0, hi = N - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (m > x[mid]) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+struct Hermite {
+    int k;
+    double h, t, h00, h01, h10, h11;  // h10, h11 include the factor h
+};
+template <int N>
+PWA_HD Hermite spl_hermite(const double* x, double m) {
+    Hermite H;
+    H.k = spl_interval_n<N>(x, m);
+    H.h = x[H.k + 1] - x[H.k];
+    H.t = (m - x[H.k]) / H.h;
+    const double u = 1.0 - H.t;
+    H.h00 = (1.0 + 2.0 * H.t) * u * u;
+    H.h01 = H.t * H.t * (3.0 - 2.0 * H.t);
+    H.h10 = H.h * H.t * u * u;
+    H.h11 = -H.h * H.t * H.t * u;
+    return H;
 }
 
 // ------------------------------------------------- triangle cusp (Pc chain)

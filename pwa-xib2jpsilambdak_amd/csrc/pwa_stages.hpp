@@ -376,6 +376,128 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
     }
 }
 
+// ---------------------------------------------------------- k_adj_pcx
+// Pc splines and Pc revised K-matrix members (only models with them): their
+// parameter rows, after k_shape on the same blocks and events.  Per sub-bin
+// the Pc lineshapes and their adjoints P_r = dF/dL_r (pc_pfactors, as in
+// k_shape) are recomputed and contracted with the cardinal splines of the
+// knot values (Hermite form with the slope matrix) or with the class's
+// lineshape duals (each member's mass and width in turn); the rows are
+// emitted per sub-bin and added into k_shape's gradient row of each block.
+template <int NZ>
+__global__ void __launch_bounds__(kBlock, 1) k_adj_pcx(const double* __restrict__ ev, int64_t ld, SegView sv,
+                                                   const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
+                                                   const double* __restrict__ pcs, const double* __restrict__ awp,
+                                                   GradLayout G, int NGp, double* __restrict__ gpart, int rpb) {
+    if constexpr (NZ > 0) {
+        extern __shared__ double lds[];
+        Emitter em = Emitter::make(lds, NGp);
+        const SegDesc sd = sv.seg();
+        const int scheme = sd.scheme;
+        const int nb = S->scheme_nbins[scheme];
+        const double inb = 1.0 / nb;
+        const int64_t n = sd.n;
+        const int zbase = G.km(PWA_KM_NCLASS) + 2 * PWA_NUMSP * S->nspl;
+        for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
+            const int64_t gi = base + threadIdx.x;
+            const bool valid = gi < n;
+            const int64_t i = sd.begin + (valid ? gi : n - 1);
+            const double mjp = ld_d(ev, ld, i, E_MJPSIP);
+            C64 p0[NZ], Mo[NZ][NZ];
+            double Md[NZ];
+#pragma unroll
+            for (int r = 0; r < NZ; ++r) {
+                p0[r] = C64(pcs[(int64_t)(2 + 2 * r) * ld + i], pcs[(int64_t)(3 + 2 * r) * ld + i]);
+                Md[r] = pcs[(int64_t)(2 + 2 * NZ + r) * ld + i];
+#pragma unroll
+                for (int q = 0; q < r; ++q) {
+                    const int o = 2 + 3 * NZ + 2 * mo_index(q, r);
+                    Mo[q][r] = C64(pcs[(int64_t)o * ld + i], pcs[(int64_t)(o + 1) * ld + i]);
+                }
+            }
+            const double sc = valid ? 2.0 * awp[i] * inb : 0.0;
+            const double sig = S->mres[0] + mjp * (S->mres[1] + mjp * S->mres[2]);
+            for (int b = 0; b < nb; ++b) {
+                const double mb = mjp + S->scheme_z[scheme][b] * sig;
+                C64 L[NZ], Pr[NZ];
+                pc_lineshapes<NZ, 2>(S, D, mb, L);
+                pc_pfactors<NZ>(L, p0, Md, Mo, Pr);
+#pragma unroll
+                for (int r = 0; r < NZ; ++r) Pr[r] = Pr[r] * sc;
+                // knot values: dF/dRe y_j = Re(P B c_j), dF/dIm y_j = -Im(P B c_j)
+                for (int s = 0; s < S->nsplz; ++s) {
+                    int rr = S->nR;
+                    for (int q = S->nR; q < S->nR + S->nZ; ++q)
+                        if (S->splz_of[q] == s) rr = q;
+                    const SplTabZ& T = D->splz[s];
+                    const bool in = mb >= T.xmin && mb <= T.xmax;
+                    const Hermite H = spl_hermite<PWA_NUMSPZ>(T.x, in ? mb : T.x[0]);
+                    const double B = in ? vertex_pc<double>(S->P, S->spin2[rr], mb, -0.5) : 0.0;
+                    const C64 w = select_nz<NZ>(Pr, rr - S->nR) * B;
+                    const double* s0 = T.sm + H.k * PWA_NUMSPZ;
+                    const double* s1 = s0 + PWA_NUMSPZ;
+                    for (int c = 0; c < PWA_NUMSPZ; c += 8) {
+                        const int nk = PWA_NUMSPZ - c < 8 ? PWA_NUMSPZ - c : 8;
+                        em.begin(2 * nk);
+                        for (int jj = 0; jj < nk; ++jj) {
+                            const int j = c + jj;
+                            double cj = s0[j] * H.h10 + s1[j] * H.h11;
+                            cj += (j == H.k) ? H.h00 : 0.0;
+                            cj += (j == H.k + 1) ? H.h01 : 0.0;
+                            em.put(2 * jj, cj * w.re);
+                            em.put(2 * jj + 1, -cj * w.im);
+                        }
+                        em.commit(2 * nk, zbase + 2 * PWA_NUMSPZ * s + 2 * c);
+                    }
+                }
+                // revised K-matrix members: dF/dp_j = sum_r Re(P_r dL_r/dp_j)
+                for (int c = S->rkm_nlam; c < S->rkm_ncls; ++c) {
+                    const int nm = S->rkm_nmem[c];
+                    double mv[kRkmMem], gv[kRkmMem];
+                    uint32_t Lpk = 0, Rpk = 0;
+#pragma unroll
+                    for (int q = 0; q < kRkmMem; ++q) {
+                        const int rq = S->rkm_mem[c][q < nm ? q : 0];
+                        Rpk |= (uint32_t)(rq & 0xff) << (8 * q);
+                        mv[q] = D->m0[rq];
+                        gv[q] = D->g0[rq];
+                        Lpk |= (uint32_t)(S->L[rq] & 0xff) << (8 * q);
+                    }
+                    for (int j = 0; j < nm; ++j) {
+                        D2 m[kRkmMem], g[kRkmMem];
+#pragma unroll
+                        for (int q = 0; q < kRkmMem; ++q) {
+                            const double on = (double)(q == j);
+                            m[q] = D2(mv[q]);
+                            m[q].d[0] = on;
+                            g[q] = D2(gv[q]);
+                            g[q].d[1] = on;
+                        }
+                        double d0 = 0.0, d1 = 0.0;
+#pragma unroll 1
+                        for (int r = 0; r < nm; ++r) {
+                            const int res = (int)((Rpk >> (8 * r)) & 0xffu);
+                            const Cx<D2> Lr = rkm_amp<D2>(S->P, true, m, g, Lpk, nm, r, mb) *
+                                              vertex_pc<D2>(S->P, S->spin2[res], mb, sel4(m, r));
+                            const C64 P = select_nz<NZ>(Pr, res - S->nR);
+                            d0 += re_mul(cder(Lr, 0), P);
+                            d1 += re_mul(cder(Lr, 1), P);
+                        }
+                        em.begin(2);
+                        em.put(0, d0);
+                        em.put(1, d1);
+                        em.commit(2, G.res((int)((Rpk >> (8 * j)) & 0xffu)));
+                    }
+                }
+            }
+        }
+        if (em.cnt > 0) em.flush();
+        __syncthreads();
+        for (int q = threadIdx.x; q < G.ng; q += kBlock)
+            gpart[((int64_t)rpb * blockIdx.x) * G.ng + q] += Emitter::block_sum(lds, NGp, q);
+    }
+}
+
 // ---------------------------------------------------------- k_pcdirect
 // Unsmeared events (lineshapes at the event's own m(J/psi p)) when every Pc
 // lineshape has the factorised form: the Pc chain is contracted once,
@@ -561,6 +683,14 @@ hipError_t run_stages_nz(const StageArgs& a, hipStream_t st) {
     }
 #undef PWA_SHAPE
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if constexpr (NZ > 0) {
+        if (a.grad && a.pcx) {
+            if ((e = lds_attr((const void*)k_adj_pcx<NZ>)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_adj_pcx<NZ>), dim3(sblocks), dim3(kBlock), lds, st, a.ev, a.ld, a.sv, a.S, a.D, pcs,
+                               awp, G, NGp, a.gpart, a.rpb);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
     if ((e = mark(ST_DIRECT)) != hipSuccess) return e;
     if constexpr (NZ > 0) {
         if (a.sv.direct_ok) {

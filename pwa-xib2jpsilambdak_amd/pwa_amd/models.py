@@ -427,6 +427,39 @@ def config_triangle(seed=7, nbins=40) -> Model:
     return m
 
 
+def config_pc_exotic(seed=8, nbins=40) -> Model:
+    """Pc-chain spline and revised K-matrix: a 1/2- Pc spline on 11 knots in
+    4.05-4.55 GeV (synthetic positions; as RooTDalitz/ResZ.h:372-383 does,
+    knots below 4.2 GeV and the last knot are held at zero), a 3/2- revised
+    K-matrix class (4.40, 4.46 GeV and a non-resonant member), a Pc
+    Breit-Wigner, 8 Breit-Wigner Lambda* and 1 Zcs, with 40-bin smearing.
+    Knot values, member masses (not the non-resonant one's) and widths float."""
+    rng = np.random.default_rng(seed)
+    d = base_desc()
+    _nominal_smearing(d, nbins)
+    res = [lambda_res(rng, s2, p, rng.uniform(1.50, 2.45), rng.uniform(0.02, 0.30))
+           for s2, p in [(1, -1), (3, -1), (3, 1), (5, -1), (5, 1), (7, 1), (1, 1), (9, 1)]]
+    res.append(pc_res(rng, 1, -1, 4.312, 0.0098))
+    kx = np.linspace(4.05, 4.55, abi.PWA_NUMSPZ)
+    kv = 0.5 * (rng.standard_normal(len(kx)) + 1j * rng.standard_normal(len(kx)))
+    kv[(kx < 4.2) | (np.arange(len(kx)) == len(kx) - 1)] = 0.0
+    spl = pc_spline_res(rng, 1, -1, kx, kv)
+    for i, x in enumerate(kx):
+        if x < 4.2 or i == len(kx) - 1:
+            spl.floating[8 + 3 * i] = spl.floating[9 + 3 * i] = False
+    res.append(spl)
+    rk = abi.PWA_REVISEDKMATRIX
+    for m0, g0 in [(4.40, 0.03), (4.46, 0.015), (-1.0, 0.2)]:
+        r = pc_res(rng, 3, -1, m0, g0, rk)
+        r.floating[12] = m0 > 0
+        r.floating[13] = True
+        res.append(r)
+    res.append(zcs_res(rng, 2, 1, 4.003, 0.131))
+    m = assemble(d, res, "pc_exotic: 8 Lambda* + Pc BW + Pc spline + 3-member Pc revised K-matrix + 1 Zcs")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
     """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
     m = np.asarray(mjpsip)

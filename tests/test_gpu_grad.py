@@ -124,6 +124,12 @@ def test_grad_pc_triangle_cusp():
     assert np.isfinite(g[o + 12]) and np.isfinite(g[o + 13]) and g[o + 12] != 0.0 and g[o + 13] != 0.0
 
 
+def test_grad_pc_spline_and_revised_kmatrix():
+    """Pc spline knot values and Pc revised K-matrix member masses / widths
+    (k_adj_pcx), with every other parameter of the model."""
+    _check_fd(models.config_pc_exotic(8), 300, 600, 47)
+
+
 def test_grad_lambda_spline():
     """Knot-value gradient of the Lambda* splines (cardinal basis from the
     slope matrix) and the rest of the spline model."""

@@ -112,3 +112,9 @@ def test_pc_triangle_cusp():
     reference's complex branch rules) with smeared Pc Breit-Wigners."""
     _check(models.config_triangle(7), 1000, 2000, 27)
 
+
+def test_pc_spline_and_revised_kmatrix():
+    """A Pc spline and a Pc revised K-matrix class (their lineshapes in the
+    smeared sub-bins and the unsmeared events) beside a Pc Breit-Wigner."""
+    _check(models.config_pc_exotic(8), 1000, 2000, 28)
+

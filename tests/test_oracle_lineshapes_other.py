@@ -300,10 +300,10 @@ def test_triangle_pc_lineshape_is_cusp_times_vertex():
 # --------------------------------------------------------------- library
 @pytest.mark.parametrize("case", ["lambda_spline", "pc_spline", "revised_km", "pc_revised_km", "triangle"])
 def test_library_layout_and_acceptance(case):
-    """The library counts every type's parameters as the oracle does; it
-    evaluates Lambda* splines, Lambda* revised K-matrix members and the Pc
-    triangle cusp, and rejects the Pc-chain spline and revised K-matrix
-    (PWA_E_MODEL)."""
+    """The library counts every type's parameters as the oracle does and
+    accepts every type on the chains where the reference evaluates it
+    (spline and revised K-matrix: Lambda* and Pc; triangle cusp: Pc); on a
+    Zcs resonance it rejects them (PWA_E_MODEL)."""
     rng = np.random.default_rng(13)
     if case == "lambda_spline":
         r = models.lambda_spline_res(rng, 1, -1, np.linspace(1.45, 2.5, abi.PWA_NUMSP), np.ones(abi.PWA_NUMSP, complex))
@@ -320,10 +320,16 @@ def test_library_layout_and_acceptance(case):
     assert _lib.lib().pwa_fit_param_count(C.byref(m.desc), C.byref(n)) == abi.PWA_OK
     assert n.value == len(m.params) == oracle.lib().orc_fit_param_count(oracle.dref(m.desc))
     rc = _lib.lib().pwa_helicity_couplings(C.byref(m.desc), None, None, C.byref(n))
-    if case in ("lambda_spline", "revised_km", "triangle"):
-        assert rc == abi.PWA_OK
-        got = _lib.helicity_couplings(m.desc, m.params)
-        ref = oracle.build_amp(m.desc, m.params)["vamp"]
-        assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
-    else:
-        assert rc == abi.PWA_E_MODEL
+    assert rc == abi.PWA_OK
+    got = _lib.helicity_couplings(m.desc, m.params)
+    ref = oracle.build_amp(m.desc, m.params)["vamp"]
+    assert np.max(np.abs(got - ref)) < 1e-13 * np.max(np.abs(ref))
+    # on a Zcs resonance they are rejected
+    d = m.desc
+    z = models.zcs_res(rng, 2, 1, 4.0, 0.1)
+    m2 = models.assemble(models.base_desc(), [models.lambda_res(rng, 3, -1, 1.52, 0.016), z])
+    m2.desc.type[1] = {"lambda_spline": abi.PWA_MODELINDEPENDENT, "pc_spline": abi.PWA_MODELINDEPENDENT,
+                       "revised_km": abi.PWA_REVISEDKMATRIX, "pc_revised_km": abi.PWA_REVISEDKMATRIX,
+                       "triangle": abi.PWA_TRIANGLE}[case]
+    assert _lib.lib().pwa_helicity_couplings(C.byref(m2.desc), None, None, C.byref(n)) == abi.PWA_E_MODEL
+    assert d.n_res == 2
