@@ -45,12 +45,11 @@ extern "C" {
 #define PWA_BREITWIGNER 1
 #define PWA_KMATRIX 240
 #define PWA_NONRESONANT 361
-/* Types with a defined parameter layout that this library does not evaluate:
-   pwa_create rejects a model that uses them with PWA_E_MODEL.  The spline
-   (MODELINDEPENDENT, Lambda* and Pc chains), revised K-matrix (REVISEDKMATRIX,
-   Lambda* and Pc) and triangle cusp (TRIANGLE, Pc) lineshapes of
-   src/kernel_calEva.cu:133-155, :404-452 are used by none of the shipped fit
-   configurations.                                                         */
+/* The spline (MODELINDEPENDENT: Lambda* and Pc chains, at most 2 each),
+   triangle cusp (TRIANGLE: Pc chain) and revised K-matrix (REVISEDKMATRIX:
+   Lambda* and Pc chains, at most 4 J^P classes of at most 4 members) of
+   src/kernel_calEva.cu:133-155, :404-452, used by none of the shipped fit
+   configurations; other codes are rejected with PWA_E_MODEL.              */
 #define PWA_MODELINDEPENDENT 3
 #define PWA_TRIANGLE 7
 #define PWA_REVISEDKMATRIX 360
