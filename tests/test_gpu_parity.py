@@ -118,3 +118,22 @@ def test_pc_spline_and_revised_kmatrix():
     smeared sub-bins and the unsmeared events) beside a Pc Breit-Wigner."""
     _check(models.config_pc_exotic(8), 1000, 2000, 28)
 
+
+
+def test_nll_contributions_from_exported_weights():
+    """The decomposition of python/calculate_nll_contributions.py on the
+    per-event weights pwa_eval exports adds up to alpha (NLL - NLL_SHIFT)."""
+    from pwa_amd.nllparts import nll_contributions_at
+    model = models.config_nominal(2)
+    model.desc.nll_shift = 1234.5
+    dcols, sw, ds, mcols, wmc, ms = _sample(model, 800, 1600, 29)
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        nll, _ = c.eval(model.params)
+        parts = nll_contributions_at(c, model.params, sw, wmc)
+        alpha = c.alpha()
+    print(f"[nllparts] data {parts['data']:.10f} mc {parts['mc']:.10f} total {parts['total']:.10f} "
+          f"alpha (nll - shift) {alpha * (nll - 1234.5):.10f}", flush=True)
+    assert abs(parts["alpha"] - alpha) < 1e-14 * alpha
+    assert abs(parts["total"] - alpha * (nll - 1234.5)) < 1e-10 * abs(parts["total"])

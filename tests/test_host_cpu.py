@@ -175,3 +175,27 @@ def test_generator_kinematics():
         assert np.all(np.abs(cols[abi.COL[c]]) <= 1 + 1e-12)
     # cosTheta_Jpsi is uniform for isotropic J/psi decay
     assert abs(np.mean(cols[abi.COL["cosTheta_Jpsi"]])) < 0.03
+
+
+# ------------------------------------------------ NLL decomposition
+def test_nll_contributions_closed_form():
+    """python/calculate_nll_contributions.py:24-33 on arrays with closed-form
+    sums: constant amp2 c (data) and c' (MC), alpha = W / sum(w^2); events
+    with amp2 <= 0 contribute nothing to the data sum."""
+    from pwa_amd.nllparts import nll_contributions
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.5, 1.5, 200)
+    wm = rng.uniform(0.8, 1.2, 500)
+    c, c2 = 2.5, 0.7
+    amp2 = np.full(200, c)
+    amp2[:7] = [0.0, -1.0, -3.0, 0.0, -1e-9, -2.0, 0.0]
+    r = nll_contributions(amp2, w, wm * c2, wm)
+    W, alpha = w.sum(), w.sum() / (w ** 2).sum()
+    assert abs(r["alpha"] - alpha) < 1e-15 * alpha
+    assert abs(r["data"] - alpha * w[7:].sum() * np.log(c)) < 1e-12 * abs(r["data"])
+    assert abs(r["mc"] - alpha * W * np.log(c2)) < 1e-12 * abs(r["mc"])
+    assert r["total"] == r["mc"] - r["data"]
+    # scaling every MC amp2 by k shifts only the MC term, by alpha W ln k
+    r2 = nll_contributions(amp2, w, 3.0 * wm * c2, wm)
+    assert abs((r2["mc"] - r["mc"]) - alpha * W * np.log(3.0)) < 1e-12 * abs(r["mc"]) + 1e-12
+    assert r2["data"] == r["data"]
