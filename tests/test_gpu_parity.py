@@ -137,3 +137,32 @@ def test_nll_contributions_from_exported_weights():
           f"alpha (nll - shift) {alpha * (nll - 1234.5):.10f}", flush=True)
     assert abs(parts["alpha"] - alpha) < 1e-14 * alpha
     assert abs(parts["total"] - alpha * (nll - 1234.5)) < 1e-10 * abs(parts["total"])
+
+
+def test_resonance_weights_match_oracle_submodels():
+    """Per-group weights (storeResonanceWeights counterpart): 'amp2' is the
+    fitted per-event |A|^2; a group equals the oracle's |A|^2 of the model
+    made of that group's resonances alone, data and (weighted) MC."""
+    from pwa_amd.weights import resonance_weights
+    model = models.config_nominal(2)
+    dcols, sw, ds, mcols, wmc, ms = _sample(model, 600, 1200, 30)
+    with Context(model.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        _, _, fxd, fxm = c.eval(model.params, want_fx=True)
+        W = resonance_weights(c, model.desc, model.params)
+    assert np.array_equal(W["amp2"][0], fxd) and np.array_equal(W["amp2"][1], fxm)
+    d = model.desc
+    for name in ("allLz", "allPc", "Pc_0", "LZ_JP_3m", "allLz_Pc_JP_1m", "Zcs_0"):
+        keep = {"allLz": range(d.n_lambda), "allPc": range(d.n_lambda, d.n_lambda + d.n_pc)}.get(name)
+        if keep is None:
+            from pwa_amd.weights import resonance_groups
+            keep = resonance_groups(d, model.params)[name]
+        sub_desc = abi.ModelDesc.from_buffer_copy(d)
+        sub = models.assemble(sub_desc, [model.resonances[r] for r in keep])
+        ref_d = oracle.amp2(sub.desc, sub.params, dcols, scheme=ds)
+        ref_m = oracle.amp2(sub.desc, sub.params, mcols, scheme=ms) * wmc
+        rel = max(np.max(np.abs(W[name][0] - ref_d)) / np.max(np.abs(ref_d)),
+                  np.max(np.abs(W[name][1] - ref_m)) / np.max(np.abs(ref_m)))
+        print(f"[weights] {name}: {len(list(keep))} resonances, max rel diff {rel:.3e}", flush=True)
+        assert rel < 1e-11

@@ -199,3 +199,29 @@ def test_nll_contributions_closed_form():
     r2 = nll_contributions(amp2, w, 3.0 * wm * c2, wm)
     assert abs((r2["mc"] - r["mc"]) - alpha * W * np.log(3.0)) < 1e-12 * abs(r["mc"]) + 1e-12
     assert r2["data"] == r["data"]
+
+
+def test_resonance_groups_and_exclusion():
+    """Group construction of storeResonanceWeights and the coupling switch:
+    a resonance outside the kept set gets all-zero helicity couplings, the
+    kept ones keep theirs."""
+    from pwa_amd.weights import exclude, resonance_groups
+    m = models.config_nominal(2)
+    d = m.desc
+    g = resonance_groups(d, m.params)
+    nl, npc, nx = d.n_lambda, d.n_pc, d.n_zcs
+    assert g["amp2"] == list(range(d.n_res))
+    assert g["allPc"] == list(range(nl, nl + npc))
+    assert sum(len(v) for k, v in g.items() if k.startswith("LZ_JP_")) == nl
+    assert all(k in g for k in ("Ls_0", "Pc_0", "Zcs_0", "LZ_high_J", "allLz_Pc_JP_1m"))
+    keep = [0, 3, nl + 1, nl + npc]
+    full = oracle.build_amp(d, m.params)
+    part = oracle.build_amp(d, exclude(d, m.params, keep))
+    ac = full["ampcount"]
+    for r in range(d.n_res):
+        hi = ac[r + 1] if r + 1 < d.n_res else len(full["vamp"])
+        a, b = full["vamp"][ac[r]:hi], part["vamp"][ac[r]:hi]
+        if r in keep:
+            assert np.array_equal(a, b)
+        else:
+            assert np.all(b == 0) and np.any(a != 0)
