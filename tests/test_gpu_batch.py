@@ -34,9 +34,17 @@ def _points(model, n, seed):
     return pts
 
 
-@pytest.mark.parametrize("cfg", ["small", "nominal"])
+_CFG = {"small": lambda: models.config_small(1), "nominal": lambda: models.config_nominal(2),
+        "spline": lambda: models.config_spline(5), "revised": lambda: models.config_revised(6),
+        "pc_exotic": lambda: models.config_pc_exotic(8)}
+
+
+@pytest.mark.parametrize("cfg", ["small", "nominal", "spline", "revised", "pc_exotic"])
 def test_toy_batch_matches_single_data_sets(cfg):
-    model = models.config_small(1) if cfg == "small" else models.config_nominal(2)
+    """Also the models whose gradient needs a second pass (Lambda* splines,
+    revised K-matrix classes, Pc splines and revised K-matrix): their
+    per-sample buffers are reused across the toys and points of a batch."""
+    model = _CFG[cfg]()
     sizes = [700, 0, 1300, 45, 400]  # uneven, one empty data set
     cols, sw, sch = _toys(model, sizes, 31)
     mcols, mw, msch = _mc(model, 3000, 32)
