@@ -39,17 +39,6 @@ static double c2(int iS, int iL) {
 static const int kHel1Two[6] = {1, 1, -1, -1, 3, -3};
 static const int kHel2[6] = {1, 0, 0, -1, 1, -1};
 
-static int param_offset(const pwa_model_desc& d, int r) {
-    int off = 0;
-    for (int q = 0; q < r; ++q) {
-        const bool spline = d.type[q] == PWA_MODELINDEPENDENT;
-        if (q < d.n_lambda) off += spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
-        else if (q < d.n_lambda + d.n_pc) off += spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
-        else off += PWA_NPAR_ZCS;
-    }
-    return off;
-}
-
 bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {
     c.clear();
     ampcount.clear();
@@ -79,7 +68,7 @@ bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {
                             ++ib;
                         }
                     // overall scale (scRe, scIm) at size - 7, after any spline knots
-                    h.b = off + 12 + (d.type[r] == PWA_MODELINDEPENDENT ? 3 * PWA_NUMSP : 0);
+                    h.b = off + param_size(d, r) - 7;
                     h.b_scale = true;
                     c.push_back(h);
                 }

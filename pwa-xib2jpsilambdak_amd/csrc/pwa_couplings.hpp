@@ -22,6 +22,21 @@ namespace pwa {
 
 using cplx = std::complex<double>;
 
+// Fit-parameter layout: the RooArgList length of resonance r (a spline
+// resonance also holds its knot triples, src/RooDalitzAmplitude.cxx:1721-1750)
+// and the offset of its list in the concatenated vector.
+inline int param_size(const pwa_model_desc& d, int r) {
+    const bool spline = d.type[r] == PWA_MODELINDEPENDENT;
+    if (r < d.n_lambda) return spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
+    if (r < d.n_lambda + d.n_pc) return spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
+    return PWA_NPAR_ZCS;
+}
+inline int param_offset(const pwa_model_desc& d, int r) {
+    int off = 0;
+    for (int q = 0; q < r; ++q) off += param_size(d, q);
+    return off;
+}
+
 struct CouplingTerm {
     int a;          // index of Re Z in the fit-parameter vector
     double alpha;   // real coefficient

@@ -133,19 +133,6 @@ int orbital_L(const pwa_model_desc& d, int r) {
     return LR;
 }
 
-// RooArgList length of resonance r (a spline resonance also holds its knot
-// triples, src/RooDalitzAmplitude.cxx:1721-1750).
-int param_size(const pwa_model_desc& d, int r) {
-    const bool spline = d.type[r] == PWA_MODELINDEPENDENT;
-    if (r < d.n_lambda) return spline ? PWA_NPAR_LAMBDA_SPLINE : PWA_NPAR_LAMBDA;
-    if (r < d.n_lambda + d.n_pc) return spline ? PWA_NPAR_PC_SPLINE : PWA_NPAR_PC;
-    return PWA_NPAR_ZCS;
-}
-int param_offset(const pwa_model_desc& d, int r) {
-    int off = 0;
-    for (int q = 0; q < r; ++q) off += param_size(d, q);
-    return off;
-}
 
 int km_class_of(int S2, int LR) { return (LR > 0) ? ((S2 == 3) ? 3 : 2) : ((S2 == 3) ? 1 : 0); }
 
