@@ -1,9 +1,9 @@
-"""The nominal workload at its full size (300k accept-reject data, 3M
-phase-space MC, 216 floating parameters) checked through properties that do
-not depend on the size: repeated and value-only evaluations agree bit for
-bit, the NLL is invariant under a global complex rescaling of the couplings
-and under a rescaling of the MC weights, and the analytic gradient matches a
-central difference along a random direction."""
+"""The nominal and K-matrix workloads at their full size (300k accept-reject
+data, 3M phase-space MC; 216 and 246 floating parameters) checked through
+properties that do not depend on the size: repeated and value-only
+evaluations agree bit for bit, the NLL is invariant under a global complex
+rescaling of the couplings and under a rescaling of the MC weights, and the
+analytic gradient matches a central difference along a random direction."""
 import os
 import sys
 from types import SimpleNamespace
@@ -21,10 +21,10 @@ from test_oracle_pins import _scale_all_couplings  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def workload():
-    model, p, dcols, mcols, _ = bench.build_workload(300_000, 3_000_000)
-    print(f"[fullsize] {dcols.shape[1]} data, {mcols.shape[1]} MC events", flush=True)
+@pytest.fixture(scope="module", params=["nominal", "kmatrix"])
+def workload(request):
+    model, p, dcols, mcols, _ = bench.build_workload(300_000, 3_000_000, name=request.param)
+    print(f"[fullsize] {request.param}: {dcols.shape[1]} data, {mcols.shape[1]} MC events", flush=True)
     return model, p, dcols, mcols
 
 
