@@ -532,8 +532,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_pcdirect(const double* __restrict
         const C64* cpl_i = opaque(cpl);
         const ChainKin kin = chain_kin(S->P, CH_PC, mjp);
         C64 A[8];
-        const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
         {
+            const ChainAng za = load_chain(ev, ld, i, E_Z_CB);
             LsEv e;
             e.key = -1;
             C64 V[2][3][2];
@@ -570,12 +570,14 @@ __global__ void __launch_bounds__(kBlock, 2) k_pcdirect(const double* __restrict
             C64 T0[2][3][2];
 #pragma unroll
             for (int h = 0; h < 8; ++h) A[h] = conj(A[h]);
-            chain_adjoint(za, A, T0);
-            // lineshapes again, now with their derivatives
+            chain_adjoint(load_chain(opaque(ev), ld, i, E_Z_CB), A, T0);
+            // lineshapes again, now with their derivatives (a runtime trip
+            // count: a single Pc would otherwise be inlined and scheduled into
+            // the adjoint contraction above, past the register budget)
             LsEv e;
             e.key = -1;
 #pragma unroll 1
-            for (int r = 0; r < NZ; ++r) {
+            for (int r = 0; r < S->nZ; ++r) {
                 const LsConst& lc = D->ls[S->nR + r];
                 ls_event_cached(S->P, kin, lc, e);
                 C64 L, dL0, dL1;
