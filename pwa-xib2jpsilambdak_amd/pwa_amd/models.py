@@ -460,6 +460,24 @@ def config_pc_exotic(seed=8, nbins=40) -> Model:
     return m
 
 
+def config_max(seed=9) -> Model:
+    """Size limits: 40 Lambda* (every 2J <= 9 and parity, several each), the
+    maximum of 8 Pc Breit-Wigners (2J <= 7, both parities), 8 Zcs (2J = 0, 2,
+    4) = 56 of the 64 resonances, with the reference's 30/150/20-sub-bin
+    schemes and resolution (use reference_schemes for the scheme ids)."""
+    rng = np.random.default_rng(seed)
+    d = reference_preset(flags=0, nll_shift=0.0)
+    jps = [(j2, p) for j2 in (1, 3, 5, 7, 9) for p in (-1, 1)]
+    res = [lambda_res(rng, j2, p, rng.uniform(1.45, 2.5), rng.uniform(0.02, 0.3)) for k in range(4) for j2, p in jps]
+    res += [pc_res(rng, j2, p, rng.uniform(4.2, 4.6), rng.uniform(0.005, 0.05))
+            for j2, p in [(1, -1), (1, 1), (3, -1), (3, 1), (5, -1), (5, 1), (7, -1), (7, 1)]]
+    res += [zcs_res(rng, j2, p, rng.uniform(3.9, 4.3), rng.uniform(0.05, 0.2))
+            for j2, p in [(0, -1), (0, 1), (2, -1), (2, 1), (4, -1), (4, 1), (2, -1), (2, 1)]]
+    m = assemble(d, res, "max: 40 Lambda* + 8 Pc + 8 Zcs, 30/150/20-bin schemes")
+    m.float_mask[0] = m.float_mask[1] = False
+    return m
+
+
 def window_schemes(mjpsip: np.ndarray, window=SMEAR_WINDOW, scheme_id=1) -> np.ndarray:
     """Smearing scheme per event: `scheme_id` inside the m(J/psi p) window, 0 outside."""
     m = np.asarray(mjpsip)

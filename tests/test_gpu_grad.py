@@ -130,6 +130,11 @@ def test_grad_pc_spline_and_revised_kmatrix():
     _check_fd(models.config_pc_exotic(8), 300, 600, 47)
 
 
+def test_grad_max_sizes():
+    """56 resonances (the maximum of 8 Pc), 698 floating parameters."""
+    _check_fd(models.config_max(9), 300, 600, 48)
+
+
 def test_grad_lambda_spline():
     """Knot-value gradient of the Lambda* splines (cardinal basis from the
     slope matrix) and the rest of the spline model."""

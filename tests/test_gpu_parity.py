@@ -166,3 +166,28 @@ def test_resonance_weights_match_oracle_submodels():
                   np.max(np.abs(W[name][1] - ref_m)) / np.max(np.abs(ref_m)))
         print(f"[weights] {name}: {len(list(keep))} resonances, max rel diff {rel:.3e}", flush=True)
         assert rel < 1e-11
+
+
+def test_max_sizes():
+    """56 resonances (40 Lambda* of every spin and parity, the maximum of 8
+    Pc, 8 Zcs) with the reference's 30/150/20-sub-bin schemes and event
+    classification."""
+    m = models.config_max(9)
+    dcols = generate_phsp(1200, 23)
+    dcols = dcols[:, models.reference_order(dcols[abi.COL["mjpsiL"]])]
+    mcols = generate_phsp(2400, 24)
+    ds, ms, _, _ = models.reference_schemes(dcols[abi.COL["mjpsiL"]], mcols[abi.COL["mjpsiL"]])
+    sw = np.random.default_rng(6).uniform(0.5, 1.5, size=dcols.shape[1])
+    wmc = np.ones(mcols.shape[1])
+    print(f"[parity] {m.name}: {m.desc.n_res} resonances, schemes data {np.bincount(ds)} MC {np.bincount(ms)}",
+          flush=True)
+    ref = oracle.nll(m.desc, m.params, dcols, sw, ds, mcols, wmc, ms)
+    with Context(m.desc) as c:
+        c.upload(abi.PWA_DATA, dcols, sw, ds)
+        c.upload(abi.PWA_MC, mcols, wmc, ms)
+        nll, integ, fxd, fxm = c.eval(m.params, want_fx=True)
+    rel_fx = max(np.max(np.abs(fxd - ref["fx_data"]) / np.abs(ref["fx_data"])),
+                 np.max(np.abs(fxm - ref["fx_mc"]) / np.abs(ref["fx_mc"])))
+    print(f"[parity] rel_fx={rel_fx:.3e} rel_nll={abs(nll - ref['nll']) / abs(ref['nll']):.3e}", flush=True)
+    assert rel_fx < FX_RTOL
+    assert abs(nll - ref["nll"]) / abs(ref["nll"]) < NLL_RTOL
