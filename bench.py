@@ -129,6 +129,15 @@ WORKLOADS = {
                 lambda: models.config_nominal(2)),
     "kmatrix": ("kmatrix: 300k data / 3M MC, 14 Lambda* + 4-channel K-matrix Pc (2 poles) + 1 Zcs, 40-sub-bin smearing",
                 lambda: models.config_kmatrix(3)),
+    # lineshape types no shipped fit uses (their costs, not metric configurations)
+    "spline": ("spline: 300k data / 3M MC, 2 spline + 8 BW Lambda* + 3 Pc BW + 1 Zcs, 40-sub-bin smearing",
+               lambda: models.config_spline(5)),
+    "revised": ("revised: 300k data / 3M MC, 6 revised-K-matrix + 6 BW Lambda* + 3 Pc BW + 1 Zcs, 40-sub-bin smearing",
+                lambda: models.config_revised(6)),
+    "triangle": ("triangle: 300k data / 3M MC, 8 Lambda* + 2 Pc BW + 1 Pc triangle cusp + 1 Zcs, 40-sub-bin smearing",
+                 lambda: models.config_triangle(7)),
+    "pc_exotic": ("pc_exotic: 300k data / 3M MC, 8 Lambda* + Pc BW + Pc spline + 3 revised-K-matrix Pc + 1 Zcs, "
+                  "40-sub-bin smearing", lambda: models.config_pc_exotic(8)),
 }
 
 
@@ -226,7 +235,8 @@ def main():
     ap.add_argument("--nmc", type=int, default=3_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", choices=sorted(WORKLOADS), default="nominal",
-                    help="workload: the metric's nominal model (default) or its K-matrix variant")
+                    help="workload: the metric's nominal model (default), its K-matrix variant, or a model "
+                         "with the other lineshape types")
     ap.add_argument("--fd-evals", type=int, default=20, help="forward evaluations timed for the FD baseline")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration)")
