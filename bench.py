@@ -13,7 +13,7 @@ adjoint kernel).  The plain method it is compared with is the same FP64
 forward NLL kernel used for a central finite-difference gradient
 (2P + 1 forward evaluations per gradient, P = floating parameters).
 
-Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
+Prints one JSON line.  --gpus above 1 is not supported (this build runs on one GPU).
 """
 import argparse
 import json
@@ -155,8 +155,8 @@ def build_workload(nd, nm, seed=2024, name="nominal"):
 
 
 def _cpu_threads():
-    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the GPU
-    box's share), else the visible cores."""
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the host
+    cores allotted to this GPU), else the visible cores."""
     try:
         return max(int(os.environ.get("OMP_NUM_THREADS", "")), 1)
     except ValueError:
