@@ -329,25 +329,30 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
         const int nm = (g == 0) ? 4 : 6;
         for (int q = beg; q < end; ++q) {
             const int r = S->lam_idx[q];
+            const double sg = S->lam_sgn[q];
             C64 Lv, dL0, dL1;
             chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
             Lv = Lv * a2;
-            const C64* c = cpl + S->ampcount[r];
-            const int base = G.cpl(S->ampcount[r]);
+            // c[0][m] = sg c[1][m] (the two proton helicities share the LS
+            // couplings up to a parity sign), so the adjoint of c[1][m] alone,
+            // dq[1][m] + sg dq[0][m], carries both: the rows of the k = 0
+            // couplings stay zero and the coupling map's chain rule, whose
+            // factor for k = 0 is sg times that of k = 1, gives the same
+            // parameter gradient
+            const C64* c = cpl + S->ampcount[r] + nm;
+            const int base = G.cpl(S->ampcount[r] + nm);
             C64 R;
+            em.begin(12);
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                em.begin(12);
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    if (m >= nm) continue;
-                    const C64 z = Lv * dq[k][m];
-                    em.put(2 * m, z.re);
-                    em.put(2 * m + 1, z.im);
-                    cmac(R, c[k * nm + m], dq[k][m]);
-                }
-                em.commit(2 * nm, base + 2 * k * nm);
+            for (int m = 0; m < 6; ++m) {
+                if (m >= nm) continue;
+                const C64 d(fma(sg, dq[0][m].re, dq[1][m].re), fma(sg, dq[0][m].im, dq[1][m].im));
+                const C64 z = Lv * d;
+                em.put(2 * m, z.re);
+                em.put(2 * m + 1, z.im);
+                cmac(R, c[m], d);
             }
+            em.commit(2 * nm, base);
             if (!S->defer[r]) {
                 em.begin(2);
                 em.put(0, a2 * re_mul(dL0, R));
@@ -475,32 +480,33 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
         C64 Lv, dL0, dL1;
         chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
         Lv = Lv * a2;
+        // the couplings of k = 0 and k = 1 are equal: one row per pair, in
+        // the k = 0 slot (the k = 1 rows stay zero)
         const C64* c = cpl + S->ampcount[r];
         C64 R;
         int n = 0;
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+        for (int ps = 0; ps < 3; ++ps) {
+            if (2 * abs(ps - 1) > J2) continue;
+            em.begin(8);
+            const int n0 = n;
 #pragma unroll
-            for (int ps = 0; ps < 3; ++ps) {
-                if (2 * abs(ps - 1) > J2) continue;
-                em.begin(8);
-                const int n0 = n;
+            for (int ip = 0; ip < 2; ++ip)
 #pragma unroll
-                for (int ip = 0; ip < 2; ++ip)
-#pragma unroll
-                    for (int iz = 0; iz < 2; ++iz) {
-                        const int hz2 = (2 * ip - 1) + (2 * iz - 1);
-                        if (hz2 > J2 || hz2 < -J2) continue;
-                        const double w = dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dx[hz2 / 2 + 1][(1 - ps) + 1];
-                        const C64 t = T[k][ps][iz] * w;
-                        const C64 z = Lv * t;
-                        em.put(2 * (n - n0), z.re);
-                        em.put(2 * (n - n0) + 1, z.im);
-                        cmac(R, c[n], t);
-                        ++n;
-                    }
-                em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
-            }
+                for (int iz = 0; iz < 2; ++iz) {
+                    const int hz2 = (2 * ip - 1) + (2 * iz - 1);
+                    if (hz2 > J2 || hz2 < -J2) continue;
+                    const C64 t = (T[0][ps][iz] * dhalf(2 * ip - 1, -1, cP, sP) +
+                                   T[1][ps][iz] * dhalf(2 * ip - 1, 1, cP, sP)) *
+                                  dx[hz2 / 2 + 1][(1 - ps) + 1];
+                    const C64 z = Lv * t;
+                    em.put(2 * (n - n0), z.re);
+                    em.put(2 * (n - n0) + 1, z.im);
+                    cmac(R, c[n], t);
+                    ++n;
+                }
+            em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
+        }
         em.begin(2);
         em.put(0, a2 * re_mul(dL0, R));
         em.put(1, a2 * re_mul(dL1, R));
@@ -517,27 +523,28 @@ __device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict_
     pc_dz(J2, cZ, sZ, dz);
     C64 T[2][3][2];
     chain_adjoint(a, U, T);
+    // the couplings of k = 0 and k = 1 are equal: one row per pair, in the
+    // k = 0 slot (the k = 1 rows stay zero)
     int n = 0;
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int ps = 0; ps < 3; ++ps) {
+        em.begin(8);
+        const int n0 = n;
 #pragma unroll
-        for (int ps = 0; ps < 3; ++ps) {
-            em.begin(8);
-            const int n0 = n;
+        for (int iz = 0; iz < 2; ++iz)
 #pragma unroll
-            for (int iz = 0; iz < 2; ++iz)
-#pragma unroll
-                for (int ip = 0; ip < 2; ++ip) {
-                    const int n2 = 2 * (ps - 1) - (2 * ip - 1);
-                    if (n2 > J2 || n2 < -J2) continue;
-                    const double w = a2 * dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2];
-                    const C64 z = T[k][ps][iz] * w;
-                    em.put(2 * (n - n0), z.re);
-                    em.put(2 * (n - n0) + 1, z.im);
-                    ++n;
-                }
-            em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
-        }
+            for (int ip = 0; ip < 2; ++ip) {
+                const int n2 = 2 * (ps - 1) - (2 * ip - 1);
+                if (n2 > J2 || n2 < -J2) continue;
+                const C64 z = (T[0][ps][iz] * dhalf(2 * ip - 1, -1, cP, sP) +
+                               T[1][ps][iz] * dhalf(2 * ip - 1, 1, cP, sP)) *
+                              (a2 * dz[iz][(n2 + 3) / 2]);
+                em.put(2 * (n - n0), z.re);
+                em.put(2 * (n - n0) + 1, z.im);
+                ++n;
+            }
+        em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
+    }
 }
 
 // Pc resonance r of an unsmeared event, whose seed is rank one,
@@ -555,28 +562,30 @@ __device__ __forceinline__ C64 pc_direct_adjoint(const DevStatic* __restrict__ S
     const C64* c = cpl + S->ampcount[r];
     const C64 La = L * a2;
     C64 P;
+    // the couplings of k = 0 and k = 1 are equal: one row per pair, in the
+    // k = 0 slot (the k = 1 rows stay zero)
     int n = 0;
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
+    for (int ps = 0; ps < 3; ++ps) {
+        em.begin(8);
+        const int n0 = n;
 #pragma unroll
-        for (int ps = 0; ps < 3; ++ps) {
-            em.begin(8);
-            const int n0 = n;
+        for (int iz = 0; iz < 2; ++iz)
 #pragma unroll
-            for (int iz = 0; iz < 2; ++iz)
-#pragma unroll
-                for (int ip = 0; ip < 2; ++ip) {
-                    const int n2 = 2 * (ps - 1) - (2 * ip - 1);
-                    if (n2 > J2 || n2 < -J2) continue;
-                    const C64 t = T0[k][ps][iz] * (dhalf(2 * ip - 1, 2 * k - 1, cP, sP) * dz[iz][(n2 + 3) / 2]);
-                    cmac(P, t, c[n]);
-                    const C64 z = La * t;
-                    em.put(2 * (n - n0), z.re);
-                    em.put(2 * (n - n0) + 1, z.im);
-                    ++n;
-                }
-            em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
-        }
+            for (int ip = 0; ip < 2; ++ip) {
+                const int n2 = 2 * (ps - 1) - (2 * ip - 1);
+                if (n2 > J2 || n2 < -J2) continue;
+                const C64 t = (T0[0][ps][iz] * dhalf(2 * ip - 1, -1, cP, sP) +
+                               T0[1][ps][iz] * dhalf(2 * ip - 1, 1, cP, sP)) *
+                              dz[iz][(n2 + 3) / 2];
+                cmac(P, t, c[n]);
+                const C64 z = La * t;
+                em.put(2 * (n - n0), z.re);
+                em.put(2 * (n - n0) + 1, z.im);
+                ++n;
+            }
+        em.commit(2 * (n - n0), G.cpl(S->ampcount[r] + n0));
+    }
     return P;
 }
 

@@ -39,6 +39,24 @@ static double c2(int iS, int iL) {
 static const int kHel1Two[6] = {1, 1, -1, -1, 3, -3};
 static const int kHel2[6] = {1, 0, 0, -1, 1, -1};
 
+// The adjoint kernels emit one row per coupling pair (k = 0, 1): they rely on
+// the couplings of the second half of a resonance's list being those of the
+// first half times one sign s (Lambda*: a parity sign; Pc, Zcs: s = 1).
+static bool halves_match(const std::vector<HelicityCoupling>& c, size_t a0, bool lambda) {
+    const size_t half = (c.size() - a0) / 2;
+    if (2 * half != c.size() - a0 || half == 0) return false;
+    const double s = c[a0].K / c[a0 + half].K;
+    for (size_t j = 0; j < half; ++j) {
+        const HelicityCoupling &h0 = c[a0 + j], &h1 = c[a0 + half + j];
+        if (h0.K != s * h1.K || h0.b != h1.b || h0.b_scale != h1.b_scale || h0.beta0 != h1.beta0 ||
+            h0.beta1 != h1.beta1 || h0.A.size() != h1.A.size())
+            return false;
+        for (size_t t = 0; t < h0.A.size(); ++t)
+            if (h0.A[t].a != h1.A[t].a || h0.A[t].alpha != h1.A[t].alpha) return false;
+    }
+    return lambda ? (s == 1.0 || s == -1.0) : s == 1.0;
+}
+
 bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {
     c.clear();
     ampcount.clear();
@@ -146,6 +164,10 @@ bool CouplingMap::build(const pwa_model_desc& d, std::string& err) {
                             c.push_back(h);
                         }
                 }
+        }
+        if (!halves_match(c, (size_t)ampcount.back(), r < d.n_lambda)) {
+            err = "internal: the couplings of the two proton helicities differ by more than a sign";
+            return false;
         }
     }
     return true;

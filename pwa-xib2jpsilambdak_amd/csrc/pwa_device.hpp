@@ -89,6 +89,9 @@ struct DevStatic {
     // Lambda* resonances grouped by 2J (1,3,5,7,9): lam_idx[lam_start[g] .. lam_start[g+1])
     int32_t lam_start[6];
     int32_t lam_idx[PWA_MAX_RES];
+    // per position q of lam_idx: the parity sign s with c[0][m] = s c[1][m]
+    // for the couplings of the two proton helicities (k = 0, 1)
+    double lam_sgn[PWA_MAX_RES];
     // K-matrix classes in use and their channel tables
     int32_t km_nclass;
     int32_t km_class[PWA_KM_NCLASS];

@@ -246,6 +246,14 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
                 if (d.spin2[r] == 2 * g + 1 && S.L[r] == L) S.lam_idx[q++] = r;
     }
     S.lam_start[5] = q;
+    // the k = 0 couplings of a Lambda* are its k = 1 couplings times a parity
+    // sign (same LS terms, same scale; CouplingMap::build checks it): read the
+    // sign off the map
+    for (int p = 0; p < q; ++p) {
+        const int r = S.lam_idx[p], a = cm.ampcount[r];
+        const int nm = d.spin2[r] == 1 ? 4 : 6;
+        S.lam_sgn[p] = cm.c[a].K / cm.c[a + nm].K;
+    }
     S.km_nclass = 0;
     for (int k = 0; k < 4; ++k)
         if (used[k]) S.km_class[S.km_nclass++] = k;
