@@ -316,54 +316,54 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
     for (int g = 0; g < 5; ++g) {
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
-        // the group's Wigner factors folded into Q once: dq = Q dl
-        C64 dq[2][6];
-        {
-            double dl[2][6];
-            lambda_dl(g, cL, sL, dl);
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int m = 0; m < 6; ++m) dq[k][m] = Q[k][m] * dl[k][m];
-        }
         const int nm = (g == 0) ? 4 : 6;
-        for (int q = beg; q < end; ++q) {
-            const int r = S->lam_idx[q];
-            const double sg = S->lam_sgn[q];
-            C64 Lv, dL0, dL1;
-            chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
-            Lv = Lv * a2;
-            // c[0][m] = sg c[1][m] (the two proton helicities share the LS
-            // couplings up to a parity sign), so the adjoint of c[1][m] alone,
-            // dq[1][m] + sg dq[0][m], carries both: the rows of the k = 0
-            // couplings stay zero and the coupling map's chain rule, whose
-            // factor for k = 0 is sg times that of k = 1, gives the same
-            // parameter gradient
-            const C64* c = cpl + S->ampcount[r] + nm;
-            const int base = G.cpl(S->ampcount[r] + nm);
-            C64 R;
-            em.begin(12);
+        for (int h = 0; h < 2; ++h) {
+            const int b0 = h ? S->lam_split[g] : beg, b1 = h ? end : S->lam_split[g];
+            if (b0 == b1) continue;
+            // c[0][m] = sg c[1][m] over this half of the group (the two proton
+            // helicities share the LS couplings up to a parity sign), so the
+            // adjoint of c[1][m] alone, dq[m] = (Q[1][m] dl[1][m] + sg Q[0][m]
+            // dl[0][m]), carries both: the rows of the k = 0 couplings stay zero
+            // and the coupling map's chain rule, whose factor for k = 0 is sg
+            // times that of k = 1, gives the same parameter gradient
+            const double sg = S->lam_sgn[b0];
+            C64 dq[6];
+            {
+                double dl[2][6];
+                lambda_dl(g, cL, sL, dl);
 #pragma unroll
-            for (int m = 0; m < 6; ++m) {
-                if (m >= nm) continue;
-                const C64 d(fma(sg, dq[0][m].re, dq[1][m].re), fma(sg, dq[0][m].im, dq[1][m].im));
-                const C64 z = Lv * d;
-                em.put(2 * m, z.re);
-                em.put(2 * m + 1, z.im);
-                cmac(R, c[m], d);
+                for (int m = 0; m < 6; ++m) dq[m] = Q[1][m] * dl[1][m] + Q[0][m] * (sg * dl[0][m]);
             }
-            em.commit(2 * nm, base);
-            if (!S->defer[r]) {
-                em.begin(2);
-                em.put(0, a2 * re_mul(dL0, R));
-                em.put(1, a2 * re_mul(dL1, R));
-                em.commit(2, G.res(r));
-            } else if (a2 != 0.0) {
-                // spline / revised K-matrix: R replaces the consumed value in
-                // the generic slot; spline_rows / rkm_rows emit the rows after
-                double* g = gen + (int64_t)(kGenPlanes * D->ls[r].slot) * ld + i;
-                g[0] = R.re;
-                g[ld] = R.im;
+            for (int q = b0; q < b1; ++q) {
+                const int r = S->lam_idx[q];
+                C64 Lv, dL0, dL1;
+                chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
+                Lv = Lv * a2;
+                const C64* c = cpl + S->ampcount[r] + nm;
+                const int base = G.cpl(S->ampcount[r] + nm);
+                C64 R;
+                em.begin(12);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    if (m >= nm) continue;
+                    const C64 z = Lv * dq[m];
+                    em.put(2 * m, z.re);
+                    em.put(2 * m + 1, z.im);
+                    cmac(R, c[m], dq[m]);
+                }
+                em.commit(2 * nm, base);
+                if (!S->defer[r]) {
+                    em.begin(2);
+                    em.put(0, a2 * re_mul(dL0, R));
+                    em.put(1, a2 * re_mul(dL1, R));
+                    em.commit(2, G.res(r));
+                } else if (a2 != 0.0) {
+                    // spline / revised K-matrix: R replaces the consumed value in
+                    // the generic slot; spline_rows / rkm_rows emit the rows after
+                    double* g = gen + (int64_t)(kGenPlanes * D->ls[r].slot) * ld + i;
+                    g[0] = R.re;
+                    g[ld] = R.im;
+                }
             }
         }
     }

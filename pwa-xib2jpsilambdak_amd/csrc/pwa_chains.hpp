@@ -126,19 +126,32 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
     for (int g = 0; g < 5; ++g) {
         const int beg = S->lam_start[g], end = S->lam_start[g + 1];
         if (beg == end) continue;
-        double dl[2][6];
-        lambda_dl(g, cL, sL, dl);
         const int nm = (g == 0) ? 4 : 6;
-        for (int q = beg; q < end; ++q) {
-            const int r = S->lam_idx[q];
-            const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
-            const C64* c = cpl + S->ampcount[r];
+        // c[0][m] = s c[1][m] with one parity sign s per half of the group:
+        // U[m] = sum_r L_r c_r[1][m] over the half, then G[k][m] += s_k dl U
+        // (the Wigner factors dl after the loop, so that they and U are not
+        // held together across it)
+        for (int h = 0; h < 2; ++h) {
+            const int b0 = h ? S->lam_split[g] : beg, b1 = h ? end : S->lam_split[g];
+            if (b0 == b1) continue;
+            C64 U[6];
+            for (int q = b0; q < b1; ++q) {
+                const int r = S->lam_idx[q];
+                const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
+                const C64* c = cpl + S->ampcount[r] + nm;
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
+                for (int m = 0; m < 6; ++m)
+                    if (m < nm) cmac(U[m], Ls, c[m]);
+            }
+            const double sg = S->lam_sgn[b0];
+            double dl[2][6];
+            lambda_dl(g, cL, sL, dl);
 #pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    if (m < nm) rmac(G[k][m], dl[k][m], Ls * c[k * nm + m]);
-                }
+            for (int m = 0; m < 6; ++m) {
+                if (m >= nm) continue;
+                rmac(G[1][m], dl[1][m], U[m]);
+                rmac(G[0][m], sg * dl[0][m], U[m]);
+            }
         }
     }
     // A[i][j][m] = e^{i(l1 phiK + l2 phiMu)} dfunhalf(theta_Lb, 2 l0, 2(l1-l2)) dfunone(theta_Jpsi, l2, dmu)

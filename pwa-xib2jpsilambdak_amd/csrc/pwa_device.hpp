@@ -90,8 +90,10 @@ struct DevStatic {
     int32_t lam_start[6];
     int32_t lam_idx[PWA_MAX_RES];
     // per position q of lam_idx: the parity sign s with c[0][m] = s c[1][m]
-    // for the couplings of the two proton helicities (k = 0, 1)
+    // for the couplings of the two proton helicities (k = 0, 1); inside a
+    // group the +1 resonances come first, [lam_start[g], lam_split[g])
     double lam_sgn[PWA_MAX_RES];
+    int32_t lam_split[5];
     // K-matrix classes in use and their channel tables
     int32_t km_nclass;
     int32_t km_class[PWA_KM_NCLASS];

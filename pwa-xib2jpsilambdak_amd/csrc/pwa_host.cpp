@@ -236,24 +236,31 @@ void build_static(const pwa_model_desc& d, const CouplingMap& cm, DevStatic& S) 
             used[ik] = true;
         }
     }
-    // Lambda* grouped by spin, ordered by orbital momentum inside a group so
-    // consecutive resonances share the per-event lineshape factors
-    int q = 0;
-    for (int g = 0; g < 5; ++g) {
-        S.lam_start[g] = q;
-        for (int L = 0; L <= 8; ++L)
-            for (int r = 0; r < S.nR; ++r)
-                if (d.spin2[r] == 2 * g + 1 && S.L[r] == L) S.lam_idx[q++] = r;
-    }
-    S.lam_start[5] = q;
     // the k = 0 couplings of a Lambda* are its k = 1 couplings times a parity
     // sign (same LS terms, same scale; CouplingMap::build checks it): read the
     // sign off the map
-    for (int p = 0; p < q; ++p) {
-        const int r = S.lam_idx[p], a = cm.ampcount[r];
-        const int nm = d.spin2[r] == 1 ? 4 : 6;
-        S.lam_sgn[p] = cm.c[a].K / cm.c[a + nm].K;
+    std::vector<double> sgn(S.nR);
+    for (int r = 0; r < S.nR; ++r) {
+        const int a = cm.ampcount[r], nm = d.spin2[r] == 1 ? 4 : 6;
+        sgn[r] = cm.c[a].K / cm.c[a + nm].K;
     }
+    // Lambda* grouped by spin, inside a group by that sign (+1 first, the
+    // split at lam_split) and then by orbital momentum, so that consecutive
+    // resonances share the per-event lineshape factors
+    int q = 0;
+    for (int g = 0; g < 5; ++g) {
+        S.lam_start[g] = q;
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1) S.lam_split[g] = q;
+            for (int L = 0; L <= 8; ++L)
+                for (int r = 0; r < S.nR; ++r)
+                    if (d.spin2[r] == 2 * g + 1 && S.L[r] == L && (sgn[r] > 0) == (h == 0)) {
+                        S.lam_sgn[q] = sgn[r];
+                        S.lam_idx[q++] = r;
+                    }
+        }
+    }
+    S.lam_start[5] = q;
     S.km_nclass = 0;
     for (int k = 0; k < 4; ++k)
         if (used[k]) S.km_class[S.km_nclass++] = k;
