@@ -160,13 +160,26 @@ PWA_HD void ls_event_cached2(const Masses& P, const ChainKin k[2], const LsConst
         e[1] = ls_event(P, k[1], c.LR, c.LB, vo);
     }
 }
-PWA_HD void ls_grad2(const LsConst& c, const ChainKin k[2], const LsEv e[2], C64 v[2], C64 d0[2], C64 d1[2]) {
+// Value (as ls_value forms it) and the parts of its (m0, Gamma0)
+// derivatives: with w = 1 / (a - i b) (w = 1 for a vertex-only shape), the
+// event factor g (G, or GB for a vertex-only shape) and E,
+//   dL/dx = C_x g w + C g w^2 (-m2_x + i K_x E)       (x = m0, Gamma0),
+// so that sum_b Re(P_b dL_b/dx) = C_x A1 + C (-m2_x A2 - K_x A3) with
+//   A1 = sum_b g Re(P_b w),  A2 = sum_b g Re(P_b w^2),  A3 = sum_b g E Im(P_b w^2)
+// (ls_grad_sums), which is what the smeared-event gradient needs of the
+// derivatives.
+struct LsParts {
+    C64 w;
+    double g, gE;
+};
+PWA_HD void ls_parts2(const LsConst& c, const ChainKin k[2], const LsEv e[2], C64 v[2], LsParts p[2]) {
     if (c.kind == LS_VERTEX) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             v[j] = C64(c.C[0] * e[j].GB, 0.0);
-            d0[j] = C64(c.C[1] * e[j].GB, 0.0);
-            d1[j] = C64(c.C[2] * e[j].GB, 0.0);
+            p[j].w = C64(1.0, 0.0);
+            p[j].g = e[j].GB;
+            p[j].gE = 0.0;
         }
         return;
     }
@@ -175,16 +188,25 @@ PWA_HD void ls_grad2(const LsConst& c, const ChainKin k[2], const LsEv e[2], C64
         const double a = c.m2[0] - k[j].s;
         const double b = c.K[0] * e[j].E;
         const double inv = rcp_nr(a * a + b * b);
-        const C64 w(a * inv, b * inv);
-        const C64 w2 = w * w;
-        const double cg = c.C[0] * e[j].G;
-        const double t = cg * inv;
+        const double t = (c.C[0] * e[j].G) * inv;
         v[j] = C64(a * t, b * t);
-        const C64 dw0 = w2 * C64(-c.m2[1], c.K[1] * e[j].E);
-        const C64 dw1 = w2 * C64(-c.m2[2], c.K[2] * e[j].E);
-        d0[j] = w * (c.C[1] * e[j].G) + dw0 * cg;
-        d1[j] = w * (c.C[2] * e[j].G) + dw1 * cg;
+        p[j].w = C64(a * inv, b * inv);
+        p[j].g = e[j].G;
+        p[j].gE = e[j].G * e[j].E;
     }
+}
+// Accumulate the three sums above for one sub-bin with P-factor Pr.
+PWA_HD void ls_grad_sums(const LsParts& p, C64 Pr, double& A1, double& A2, double& A3) {
+    const C64 pw = Pr * p.w;
+    const C64 pw2 = pw * p.w;
+    A1 += p.g * pw.re;
+    A2 += p.g * pw2.re;
+    A3 += p.gE * pw2.im;
+}
+// (m0, Gamma0) derivative sums from A1..A3.
+PWA_HD void ls_grad_final(const LsConst& c, double A1, double A2, double A3, double& gm, double& gw) {
+    gm = c.C[1] * A1 - c.C[0] * (c.m2[1] * A2 + c.K[1] * A3);
+    gw = c.C[2] * A1 - c.C[0] * (c.m2[2] * A2 + c.K[2] * A3);
 }
 
 // Host: constants of resonance r for this evaluation (kind LS_GENERIC when

@@ -188,7 +188,12 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
             C64 L[NZ];
             if constexpr (GRAD && !FULL && NZ <= 3) {
             // two sub-bins per pass (the second one zero-weighted past the end);
-            // with more Pc resonances the doubled lineshape state exceeds 256 VGPRs
+            // with more Pc resonances the doubled lineshape state exceeds 256 VGPRs.
+            // The (m0, Gamma0) rows come from three sums per resonance
+            // (ls_parts2), turned into derivatives once per event.
+            double A1[NZ], A2[NZ], A3[NZ];
+#pragma unroll
+            for (int r = 0; r < NZ; ++r) A1[r] = A2[r] = A3[r] = 0.0;
             for (int b = 0; b < nb; b += 2) {
                 const bool two = b + 1 < nb;
                 ChainKin kin[2];
@@ -196,23 +201,27 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                 kin[1] = chain_kin(S->P, CH_PC, two ? mjp + S->scheme_z[scheme][b + 1] * sig : mjp);
                 LsEv e[2];
                 e[0].key = -1;
-                C64 Lj[2][NZ], dA[2][NZ], dB[2][NZ];
+                C64 Lj[2][NZ];
+                LsParts pj[2][NZ];
 #pragma unroll
                 for (int r = 0; r < NZ; ++r) {
                     const LsConst& lc = D->ls[S->nR + r];
                     ls_event_cached2(S->P, kin, lc, e);
-                    C64 v[2], d0[2], d1[2];
-                    ls_grad2(lc, kin, e, v, d0, d1);
+                    C64 v[2];
+                    LsParts pp[2];
+                    ls_parts2(lc, kin, e, v, pp);
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         Lj[j][r] = v[j];
-                        dA[j][r] = d0[j];
-                        dB[j][r] = d1[j];
+                        pj[j][r] = pp[j];
                     }
                 }
                 if (!two) {
 #pragma unroll
-                    for (int r = 0; r < NZ; ++r) Lj[1][r] = dA[1][r] = dB[1][r] = C64(0.0, 0.0);
+                    for (int r = 0; r < NZ; ++r) {
+                        Lj[1][r] = C64(0.0, 0.0);
+                        pj[1][r].g = pj[1][r].gE = 0.0;
+                    }
                 }
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
@@ -225,11 +234,12 @@ __global__ void __launch_bounds__(kBlock) k_shape(const double* __restrict__ ev,
                         B2d[r] += norm2(Lj[j][r]);
 #pragma unroll
                         for (int q = 0; q < r; ++q) B2o[q][r] = B2o[q][r] + Lj[j][q] * conj(Lj[j][r]);
-                        gm[r] += re_mul(Pr[r], dA[j][r]);
-                        gw[r] += re_mul(Pr[r], dB[j][r]);
+                        ls_grad_sums(pj[j][r], Pr[r], A1[r], A2[r], A3[r]);
                     }
                 }
             }
+#pragma unroll
+            for (int r = 0; r < NZ; ++r) ls_grad_final(D->ls[S->nR + r], A1[r], A2[r], A3[r], gm[r], gw[r]);
             } else
             for (int b = 0; b < nb; ++b) {
                 const double mb = mjp + S->scheme_z[scheme][b] * sig;
