@@ -203,7 +203,8 @@ PWA_HD void ls_grad_sums(const LsParts& p, C64 Pr, double& A1, double& A2, doubl
     A2 += p.g * pw2.re;
     A3 += p.gE * pw2.im;
 }
-// (m0, Gamma0) derivative sums from A1..A3.
+// (m0, Gamma0) derivative sums from A1..A3 (m2 and K are zero for a
+// vertex-only shape, whose sums A2 = A1 and A3 = 0 then drop out).
 PWA_HD void ls_grad_final(const LsConst& c, double A1, double A2, double A3, double& gm, double& gw) {
     gm = c.C[1] * A1 - c.C[0] * (c.m2[1] * A2 + c.K[1] * A3);
     gw = c.C[2] * A1 - c.C[0] * (c.m2[2] * A2 + c.K[2] * A3);
