@@ -43,6 +43,10 @@ __global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev,
     const ChainAng a = load_chain(ev, ld, i, E_Z_CB);
     const double cP = ld_d(ev, ld, i, E_Z_CP), sP = ld_d(ev, ld, i, E_Z_SP);
     const double cZ = ld_d(ev, ld, i, E_Z_CZ), sZ = ld_d(ev, ld, i, E_Z_SZ);
+    // H0 read once, before the S_r planes are written into the same array
+    C64 h0[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) h0[h] = C64(amp[(int64_t)(2 * h) * ld + i], amp[(int64_t)(2 * h + 1) * ld + i]);
     C64 Sr[NZ][8];
 #pragma unroll
     for (int r = 0; r < NZ; ++r) {
@@ -51,8 +55,7 @@ __global__ void __launch_bounds__(kBlock) k_pcang(const double* __restrict__ ev,
         double md = 0.0;
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
-            const C64 h0(amp[(int64_t)(2 * h) * ld + i], amp[(int64_t)(2 * h + 1) * ld + i]);
-            cmac(p0, conj(h0), Sr[r][h]);
+            cmac(p0, conj(h0[h]), Sr[r][h]);
             md += norm2(Sr[r][h]);
         }
         P(2 + 2 * r) = p0.re;
