@@ -89,11 +89,6 @@ struct DevStatic {
     // Lambda* resonances grouped by 2J (1,3,5,7,9): lam_idx[lam_start[g] .. lam_start[g+1])
     int32_t lam_start[6];
     int32_t lam_idx[PWA_MAX_RES];
-    // per position q of lam_idx: the parity sign s with c[0][m] = s c[1][m]
-    // for the couplings of the two proton helicities (k = 0, 1); inside a
-    // group the +1 resonances come first, [lam_start[g], lam_split[g])
-    double lam_sgn[PWA_MAX_RES];
-    int32_t lam_split[5];
     // K-matrix classes in use and their channel tables
     int32_t km_nclass;
     int32_t km_class[PWA_KM_NCLASS];
@@ -104,6 +99,12 @@ struct DevStatic {
     double mres[3];
     int32_t scheme_nbins[PWA_MAX_SCHEMES + 1];               // [0] = 1 (unsmeared)
     double scheme_z[PWA_MAX_SCHEMES + 1][PWA_MAX_SUBBINS];   // [0][0] = 0
+    // per position q of lam_idx: the parity sign s with c[0][m] = s c[1][m]
+    // for the couplings of the two proton helicities (k = 0, 1); inside a
+    // group the +1 resonances come first, [lam_start[g], lam_split[g]).
+    // (Last, so that the offsets of the fields above stay as they were.)
+    double lam_sgn[PWA_MAX_RES];
+    int32_t lam_split[5];
 };
 
 // Per-evaluation parameters.
