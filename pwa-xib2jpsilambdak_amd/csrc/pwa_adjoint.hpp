@@ -32,17 +32,22 @@ __device__ __forceinline__ const T* opaque(const T* p) {
     return p;
 }
 
-struct Emitter {
+// kOverlap: the flush issues its 16 LDS reads per lane before adding them
+// (16 more live VGPRs, no serial waits); kernels whose register budget has
+// the room take it, the others read and add one pair at a time.  The sum
+// order, and so the result, is the same.
+template <bool kOverlap>
+struct EmitterT {
     lds_f64* stage;  // [kRows][kLd]
     lds_i32* sidx;   // [kRows] gradient index of each staged row
     lds_f64* acc;    // [NGp] this wave's accumulator
     int cnt;         // staged rows (uniform)
     int lane;
 
-    __device__ __forceinline__ static Emitter make(double* lds_raw, int NGp) {
+    __device__ __forceinline__ static EmitterT make(double* lds_raw, int NGp) {
         lds_f64* l = (lds_f64*)lds_raw;
         const int wave = threadIdx.x >> 6;
-        Emitter e;
+        EmitterT e;
         e.stage = l + wave * (kRows * kLd);
         e.acc = l + 4 * kRows * kLd + wave * NGp;
         e.sidx = (lds_i32*)(l + 4 * kRows * kLd + 4 * NGp) + wave * kRows;
@@ -65,8 +70,16 @@ struct Emitter {
         const bool live = c < cnt;
         if (live) {
             const lds_f64* src = stage + c * kLd + q * 16;
+            if constexpr (kOverlap) {  // all 16 reads issued before the adds
+                double v[16];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) s += src[e];
+                for (int e = 0; e < 16; ++e) v[e] = src[e];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) s += v[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) s += src[e];
+            }
         }
         s += __shfl_xor(s, 16);
         s += __shfl_xor(s, 32);
@@ -88,6 +101,8 @@ struct Emitter {
         cnt += n;
     }
 };
+using Emitter = EmitterT<false>;
+using EmitterOv = EmitterT<true>;
 
 // Layout of the kernel-level gradient vector.
 struct GradLayout {
@@ -145,8 +160,9 @@ __device__ __forceinline__ void spline_generic(const DevStatic* __restrict__ S, 
 // Knot rows of a spline resonance: dF/dRe y_j = a2 Re(B c_j R), dF/dIm y_j =
 // -a2 Im(B c_j R), c_j(m) = d p(m) / d y_j the cardinal spline, from the
 // Hermite form with the slope matrix.
+template <class Em>
 __device__ __forceinline__ void spline_adjoint(const SplTab& T, C64 bt, C64 kh, double a2, C64 R, int base,
-                                               Emitter& em) {
+                                               Em& em) {
     const int k = (int)kh.re;
     const double h = kh.im, t = bt.im, u = 1.0 - t;
     const double h00 = (1.0 + 2.0 * t) * u * u, h01 = t * t * (3.0 - 2.0 * t);
@@ -282,10 +298,11 @@ __device__ __forceinline__ void pc_ls_grad(const DevStatic* __restrict__ S, cons
 }
 
 // ---------------------------------------------------------- Lambda* adjoint
+template <class Em>
 __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                                const C64* __restrict__ cpl, const double* __restrict__ ev,
                                                double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
-                                               const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
+                                               const C64 Y[8], double a2, const GradLayout& G, Em& em) {
     const double cB = ld_d(ev, ld, i, E_L_CB), sB = ld_d(ev, ld, i, E_L_SB);
     const double cJ = ld_d(ev, ld, i, E_L_CJ), sJ = ld_d(ev, ld, i, E_L_SJ);
     const double cL = ld_d(ev, ld, i, E_L_CL), sL = ld_d(ev, ld, i, E_L_SL);
@@ -374,9 +391,10 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
 // the class's lineshapes x vertex terms are evaluated with (m_j, Gamma_j)
 // as the dual variables and contracted with the members' adjoints R (left in
 // their generic slots by lambda_adjoint).
+template <class Em>
 __device__ __forceinline__ void rkm_rows(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                          const double* __restrict__ gen, int64_t ld, int64_t i, double mkp,
-                                         double a2, const GradLayout& G, Emitter& em) {
+                                         double a2, const GradLayout& G, Em& em) {
     for (int c = 0; c < S->rkm_nlam; ++c) {
         const int n = S->rkm_nmem[c];
         double mv[kRkmMem], gv[kRkmMem];
@@ -423,9 +441,10 @@ __device__ __forceinline__ void rkm_rows(const DevStatic* __restrict__ S, const 
 // Knot rows of the spline resonances (after lambda_adjoint has left each
 // one's adjoint R in its generic slot; where a2 = 0 the slot still holds the
 // finite value and the rows are zero).
+template <class Em>
 __device__ __forceinline__ void spline_rows(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                             const double* __restrict__ gen, int64_t ld, int64_t i, double a2,
-                                            const GradLayout& G, Emitter& em) {
+                                            const GradLayout& G, Em& em) {
     for (int s = 0; s < S->nspl; ++s) {
         const double* g = gen + (int64_t)(kGenPlanes * D->ls[S->spl_res[s]].slot) * ld + i;
         spline_adjoint(D->spl[s], C64(g[2 * ld], g[3 * ld]), C64(g[4 * ld], g[5 * ld]), a2, C64(g[0], g[ld]),
@@ -457,10 +476,11 @@ __device__ __forceinline__ void chain_adjoint(const ChainAng& a, const C64 W[8],
 }
 
 // ------------------------------------------------------------- Zcs adjoint
+template <class Em>
 __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D,
                                             const C64* __restrict__ cpl, const double* __restrict__ ev,
                                             const double* __restrict__ gen, int64_t ld, int64_t i, double mjk,
-                                            const C64 Y[8], double a2, const GradLayout& G, Emitter& em) {
+                                            const C64 Y[8], double a2, const GradLayout& G, Em& em) {
     const ChainAng a = load_chain(ev, ld, i, E_X_CB);
     const double cX = ld_d(ev, ld, i, E_X_CZ), sX = ld_d(ev, ld, i, E_X_SZ);
     const double cP = ld_d(ev, ld, i, E_X_CP), sP = ld_d(ev, ld, i, E_X_SP);
@@ -515,9 +535,10 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
 }
 
 // ---------------------------------------------------- Pc coupling adjoint
+template <class Em>
 __device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict__ S, int r, const ChainAng& a, double cZ,
                                                     double sZ, double cP, double sP, const C64 U[8], double a2,
-                                                    const GradLayout& G, Emitter& em) {
+                                                    const GradLayout& G, Em& em) {
     const int J2 = S->spin2[r];
     double dz[2][4];
     pc_dz(J2, cZ, sZ, dz);
@@ -553,9 +574,10 @@ __device__ __forceinline__ void pc_coupling_adjoint(const DevStatic* __restrict_
 // pc_coupling_adjoint with T = L_r T0.  Returns P_r = sum_h conj(A_h) S_rh
 // = sum_v T0_v V_rv (V_r: pc_vtable), the factor of the lineshape-parameter
 // derivatives d|A|^2 = 2 Re(dL_r P_r).
+template <class Em>
 __device__ __forceinline__ C64 pc_direct_adjoint(const DevStatic* __restrict__ S, const C64* __restrict__ cpl, int r,
                                                  const C64 T0[2][3][2], double cZ, double sZ, double cP, double sP,
-                                                 C64 L, double a2, const GradLayout& G, Emitter& em) {
+                                                 C64 L, double a2, const GradLayout& G, Em& em) {
     const int J2 = S->spin2[r];
     double dz[2][4];
     pc_dz(J2, cZ, sZ, dz);

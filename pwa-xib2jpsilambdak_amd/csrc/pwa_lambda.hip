@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_adj_lam(const double* __restrict_
                                                    const double* __restrict__ amp, const double* __restrict__ awp,
                                                    GradLayout G, int NGp, double* __restrict__ gpart, int rpb) {
     extern __shared__ double lds[];
-    Emitter em = Emitter::make(lds, NGp);
+    EmitterOv em = EmitterOv::make(lds, NGp);
     const SegDesc sd = sv.seg();
     const int64_t n = sd.n;
     for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {

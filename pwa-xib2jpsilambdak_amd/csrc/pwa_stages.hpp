@@ -530,8 +530,8 @@ __global__ void __launch_bounds__(kBlock, 2) k_pcdirect(const double* __restrict
     const SegDesc sd = sv.seg();
     if (!sv.direct(sd)) return;
     extern __shared__ double lds[];
-    Emitter em;
-    if constexpr (GRAD) em = Emitter::make(lds, NGp);
+    EmitterOv em;
+    if constexpr (GRAD) em = EmitterOv::make(lds, NGp);
     double s0 = 0, c0 = 0, s1 = 0, c1 = 0;
     const int64_t n = sd.n;
     for (int64_t base = (int64_t)(blockIdx.x - sd.blk0) * kBlock; base < n; base += (int64_t)sd.nblk * kBlock) {
