@@ -191,10 +191,9 @@ __device__ __forceinline__ C64 cder(const Cx<D2>& x, int k) { return C64(x.re.d[
 // ------------------------------------- lineshape x vertex with derivatives
 // Value and (m0, Gamma0) derivatives of a Lambda* or Zcs lineshape x vertex
 // term (factorised form, or the generic buffer).
-__device__ __forceinline__ void chain_ls_grad(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                              const ChainKin& kin, LsEv& e, const double* __restrict__ gen,
-                                              int64_t ld, int64_t i, C64& v, C64& d0, C64& d1) {
-    const LsConst& lc = D->ls[r];
+__device__ __forceinline__ void chain_ls_grad(const DevStatic* __restrict__ S, const LsConst& lc, const ChainKin& kin,
+                                              LsEv& e, const double* __restrict__ gen, int64_t ld, int64_t i, C64& v,
+                                              C64& d0, C64& d1) {
     if (lc.kind != LS_GENERIC) {
         ls_event_cached(S->P, kin, lc, e);
         ls_grad(lc, kin, e, v, d0, d1);
@@ -352,12 +351,13 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                 for (int m = 0; m < 6; ++m) dq[m] = Q[1][m] * dl[1][m] + Q[0][m] * (sg * dl[0][m]);
             }
             for (int q = b0; q < b1; ++q) {
-                const int r = S->lam_idx[q];
+                const DevDyn::LamPos& lp = D->lq[q];
+                const int r = lp.r;
                 C64 Lv, dL0, dL1;
-                chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
+                chain_ls_grad(S, lp.ls, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
                 Lv = Lv * a2;
-                const C64* c = cpl + S->ampcount[r] + nm;
-                const int base = G.cpl(S->ampcount[r] + nm);
+                const C64* c = cpl + lp.cpl0 + nm;
+                const int base = G.cpl(lp.cpl0 + nm);
                 C64 R;
                 em.begin(12);
 #pragma unroll
@@ -369,7 +369,7 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                     cmac(R, c[m], dq[m]);
                 }
                 em.commit(2 * nm, base);
-                if (!S->defer[r]) {
+                if (!lp.defer) {
                     em.begin(2);
                     em.put(0, a2 * re_mul(dL0, R));
                     em.put(1, a2 * re_mul(dL1, R));
@@ -377,7 +377,7 @@ __device__ __forceinline__ void lambda_adjoint(const DevStatic* __restrict__ S, 
                 } else if (a2 != 0.0) {
                     // spline / revised K-matrix: R replaces the consumed value in
                     // the generic slot; spline_rows / rkm_rows emit the rows after
-                    double* g = gen + (int64_t)(kGenPlanes * D->ls[r].slot) * ld + i;
+                    double* g = gen + (int64_t)(kGenPlanes * lp.ls.slot) * ld + i;
                     g[0] = R.re;
                     g[ld] = R.im;
                 }
@@ -498,7 +498,7 @@ __device__ __forceinline__ void zcs_adjoint(const DevStatic* __restrict__ S, con
         double dx[3][3];
         zcs_dx(J2, cX, sX, dx);
         C64 Lv, dL0, dL1;
-        chain_ls_grad(S, D, r, kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
+        chain_ls_grad(S, D->ls[r], kin, ev_ls, gen, ld, i, Lv, dL0, dL1);
         Lv = Lv * a2;
         // the couplings of k = 0 and k = 1 are equal: one row per pair, in
         // the k = 0 slot (the k = 1 rows stay zero)

@@ -99,10 +99,8 @@ __device__ __forceinline__ void zcs_dx(int J2, double c, double s, double dx[3][
 
 // ------------------------------------------------ lineshape x vertex values
 // Lambda* or Zcs resonance r (factorised form, or the generic buffer).
-__device__ __forceinline__ C64 chain_ls(const DevStatic* __restrict__ S, const DevDyn* __restrict__ D, int r,
-                                        const ChainKin& kin, LsEv& e, const double* __restrict__ gen, int64_t ld,
-                                        int64_t i) {
-    const LsConst& lc = D->ls[r];
+__device__ __forceinline__ C64 chain_ls(const DevStatic* __restrict__ S, const LsConst& lc, const ChainKin& kin,
+                                        LsEv& e, const double* __restrict__ gen, int64_t ld, int64_t i) {
     if (lc.kind != LS_GENERIC) {
         ls_event_cached(S->P, kin, lc, e);
         return ls_value(lc, kin, e);
@@ -136,9 +134,9 @@ __device__ __forceinline__ void lambda_chain(const DevStatic* __restrict__ S, co
             if (b0 == b1) continue;
             C64 U[6];
             for (int q = b0; q < b1; ++q) {
-                const int r = S->lam_idx[q];
-                const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
-                const C64* c = cpl + S->ampcount[r] + nm;
+                const DevDyn::LamPos& lp = D->lq[q];
+                const C64 Ls = chain_ls(S, lp.ls, kin, ev_ls, gen, ld, i);
+                const C64* c = cpl + lp.cpl0 + nm;
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
                     if (m < nm) cmac(U[m], Ls, c[m]);
@@ -292,7 +290,7 @@ __device__ __forceinline__ void zcs_chain(const DevStatic* __restrict__ S, const
         const int J2 = S->spin2[r];
         double dx[3][3];
         zcs_dx(J2, cX, sX, dx);
-        const C64 Ls = chain_ls(S, D, r, kin, ev_ls, gen, ld, i);
+        const C64 Ls = chain_ls(S, D->ls[r], kin, ev_ls, gen, ld, i);
         const C64* c = cpl + S->ampcount[r];
         int n = 0;
 #pragma unroll

@@ -126,6 +126,15 @@ struct DevDyn {
     double km_dm2g[PWA_KM_NCLASS][PWA_KM_NPOLE];
     SplTab spl[kMaxSpline];
     SplTabZ splz[kMaxSplineZ];
+    // Lambda* lineshape constants in lam_idx order (position q) with the
+    // resonance index, its first coupling and its defer flag, so that the
+    // resonance loops read one record per position instead of a chain of
+    // dependent loads (last, so that the offsets above stay as they were)
+    struct LamPos {
+        LsConst ls;
+        int32_t r, cpl0, defer, pad;
+    };
+    LamPos lq[PWA_MAX_RES];
 };
 
 // Per-block reduction record: compensated sums (hi, lo).

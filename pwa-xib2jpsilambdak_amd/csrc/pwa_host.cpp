@@ -425,6 +425,10 @@ int build_dyn(const pwa_ctx& c, const double* par, DevDyn& D) {
         }
     }
     km_pole_terms(c.hs, D);
+    for (int q = 0; q < c.hs.lam_start[5]; ++q) {
+        const int r = c.hs.lam_idx[q];
+        D.lq[q] = DevDyn::LamPos{D.ls[r], r, c.hs.ampcount[r], c.hs.defer[r], 0};
+    }
     return ngen;
 }
 
